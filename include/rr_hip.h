@@ -1,0 +1,149 @@
+/*
+ * rr_hip.h -- C-ABI of the MI355X RoadRunner transit-model library (librr_hip.so).
+ *
+ * Drop-in boundary for PyTransit's RoadRunner hot path.  Every entry point takes
+ * plain pointers and sizes, caller-owned DEVICE buffers (row-major, C order) and a
+ * hipStream_t passed as `void*` (NULL = the default stream).  No exceptions cross
+ * the ABI: each call returns RR_OK (0) or a negative error code, and
+ * rr_last_error() returns the text of the calling thread's last error.  The
+ * library keeps no state between calls apart from what the caller passes in, so
+ * calls on different streams are independent.
+ *
+ * Reference citations are to PyTransit 2.8.1 (paths relative to its checkout).
+ */
+#ifndef RR_HIP_H
+#define RR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RR_OK 0
+#define RR_EINVAL (-1)   /* bad argument (shape, size, null pointer)          */
+#define RR_EHIP (-2)     /* a HIP runtime call or kernel launch failed         */
+#define RR_ERANGE (-3)   /* a size exceeds a compiled-in limit                 */
+
+/* Arithmetic of the flux stage.  RR_FP64: every step in FP64, flux is double*.
+ * RR_MIXED: FP64 epoch/phase reduction and per-pv setup, FP32 z, area,
+ * limb-darkening interpolation and accumulation; flux is float*. */
+#define RR_FP64 0
+#define RR_MIXED 1
+
+/* Limb-darkening laws; the string keys of RoadRunnerModel.ldmodels
+ * (pytransit/models/roadrunner/rrmodel.py:48-58, laws in models/numba/ldmodels.py:22-139). */
+#define RR_LD_UNIFORM 0
+#define RR_LD_LINEAR 1
+#define RR_LD_QUADRATIC 2
+#define RR_LD_QUADRATIC_TRI 3
+#define RR_LD_NONLINEAR 4
+#define RR_LD_GENERAL 5
+#define RR_LD_SQUARE_ROOT 6
+#define RR_LD_LOGARITHMIC 7
+#define RR_LD_EXPONENTIAL 8
+#define RR_LD_POWER_2 9
+#define RR_LD_POWER_2_PM 10
+
+/* Limb-darkening weight table (common.py:188-223), device-resident. */
+typedef struct rr_table {
+    int32_t nk, ng, nz;
+    int32_t pad_;
+    double kmin, kmax;
+    double dk;               /* (kmax-kmin)/nk, the reference's indexing step (Q1)   */
+    double dg;               /* gs[1]-gs[0] of linspace(0, 1-1e-7, ng)               */
+    const double* weights;   /* [nk, ng, nz] FP64; may be NULL for TS exact weights  */
+    const double* ze;        /* [nz] annulus edges                                   */
+} rr_table;
+
+/* Light-curve data set by set_data() (transitmodel.py:56-125), device-resident. */
+typedef struct rr_lcdata {
+    int32_t npt, nlc, npb, nep;
+    const double* times;      /* [npt]                                               */
+    const int32_t* lcids;     /* [npt], values in [0, nlc)                           */
+    const int32_t* pbids;     /* [nlc], values in [0, npb)                           */
+    const int32_t* epids;     /* [nlc], values in [0, nep)                           */
+    const int32_t* nsamples;  /* [nlc], >= 1                                         */
+    const double* exptimes;   /* [nlc]                                               */
+} rr_lcdata;
+
+/* Model parameters of evaluate(k, ldc, t0, p, a, i, e, w), device-resident. */
+typedef struct rr_params {
+    int32_t npv;
+    int32_t kcols;            /* columns of k: 1 or npb                              */
+    int32_t t0cols;           /* columns of t0: nep                                  */
+    int32_t pad_;
+    const double* k;          /* [npv, kcols]                                        */
+    const double* t0;         /* [npv, t0cols]                                       */
+    const double* p;          /* [npv]                                               */
+    const double* a;          /* [npv]                                               */
+    const double* inc;        /* [npv]                                               */
+    const double* e;          /* [npv]                                               */
+    const double* w;          /* [npv]                                               */
+} rr_params;
+
+const char* rr_last_error(void);
+int rr_abi_version(void);
+
+/* Annulus edges ze[nin+nedge], means zm[nin+nedge] and mu = sqrt(1-zm^2), computed
+ * on the host.  Replaces create_z_grid (common.py:131-149) and rrmodel.py:171-172. */
+int rr_z_grid(double zcut, int nin, int nedge, double* ze, double* zm, double* mu);
+
+/* 3-D weight table W[nk, ng, nz] (FP64, device) on linspace(kmin, kmax, nk) x
+ * linspace(0, 1-1e-7, ng); optional FP32 copy W32 (NULL to skip).  Writes the
+ * reference's dk = (kmax-kmin)/nk and dg to *dk_out / *dg_out (host pointers).
+ * Replaces calculate_weights_3d (common.py:188-223), called at rrmodel.py:173. */
+int rr_weights_3d(int nk, double kmin, double kmax, const double* ze, int nz, int ng,
+                  double* W, float* W32, double* dk_out, double* dg_out, void* stream);
+
+/* Limb-darkening profiles ldp[npv, npb, nmu] at mu[nmu] and disk integrals
+ * istar[npv, npb] for a named law; ldc is [npv, npb, nldc].  Laws without an
+ * analytic integral use the 200-node trapezoid of rrmodel.py:220-227.
+ * Replaces evaluate_ld / evaluate_ldi (models/numba/ldmodels.py:142-175) as used
+ * at rrmodel.py:215-227 and tsmodel.py:103-115. */
+int rr_ld_profiles(int law, const double* ldc, int npv, int npb, int nldc,
+                   const double* mu, int nmu, double* ldp, double* istar, void* stream);
+
+/* Disk integrals of tabulated profiles, istar[npv, npb] = 2 pi trapz_z(z * ldp).
+ * Replaces integrate_profiles_set (models/numba/ldtkldm.py:77-91). */
+int rr_integrate_profiles(const double* mu, int nmu, const double* ldp, int npv, int npb,
+                          double* istar, void* stream);
+
+/* Bytes of device workspace rr_evaluate / ts_evaluate need for these sizes. */
+size_t rr_workspace_bytes(int npv, int npb, int ng, int nlc);
+
+/* The full RoadRunner model: per-pv setup (validity, Taylor orbit, contact-time
+ * window, limb-darkening means) then the flux over all (pv, time) points.
+ * flux is [npv, npt]: double* for RR_FP64, float* for RR_MIXED.
+ * ldp is [npv, npb, nz] and istar [npv, npb] (FP64).
+ * Replaces rr_full (pytransit/models/roadrunner/model_full.py:9-100) and its
+ * npv=1 specialisation rr_simple (model_simple.py:11-138). */
+int rr_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
+                const double* ldp, const double* istar, void* workspace, size_t workspace_bytes,
+                void* flux, void* stream);
+
+/* Orbit boundary alone: Taylor coefficients coeffs[npv, 2, 5] (row 0 x, row 1 y,
+ * coefficient n = d^n/dt^n / n!) and contact window win[npv, 2] = (t1, t4) for
+ * radius ratio k0[npv].  Replaces meepmeep's solve2d + bounding_box as called at
+ * model_full.py:60,65 (method: orbits/taylor_z.py:23-102,298-328,391-394). */
+int rr_orbit(int npv, const double* p, const double* a, const double* inc, const double* e,
+             const double* w, const double* k0, double* coeffs, double* win, void* stream);
+
+/* Transmission-spectroscopy model (TSModel): k is [npv, npb] (kcols = npb), t0 is
+ * [npv, 1], nsamples[0]/exptimes[0] apply to all points, lcids/pbids ignored (Q16).
+ * tab->weights == NULL selects exact per-pv 2-D weights at mean(k) (the default
+ * precompute_weights=False path).  flux is [npv, npb, npt].  With RR_MIXED the
+ * weights x profile contraction runs on FP32 MFMA.
+ * Replaces tsmodel_serial (pytransit/models/roadrunner/model_trspec.py:11-93) as
+ * called by TransmissionSpectroscopyModel.evaluate (tsmodel.py:46-130). */
+size_t ts_workspace_bytes(int npv, int npb, int ng, int nz, int precision);
+int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
+                const double* ldp, const double* istar, void* workspace, size_t workspace_bytes,
+                void* flux, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RR_HIP_H */

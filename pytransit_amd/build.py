@@ -1,0 +1,57 @@
+"""Build ``librr_hip.so`` in-tree with hipcc for gfx950.
+
+The library links against torch's bundled HIP runtime (``torch/lib``), so that at
+run time it binds the runtime torch has already loaded and torch streams can be
+handed to its entry points.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, 'csrc')
+SOURCES = ['rr_kernels.hip', 'ts_kernels.hip']
+OUT = os.path.join(HERE, 'librr_hip.so')
+
+HIPCC_FLAGS = [
+    '--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
+    # no packed-FP32 instructions anywhere in this library
+    '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+    # FP32 sqrt/div to 1 ulp (the mixed path's error budget is in ppm of flux)
+    '-fno-hip-fp32-correctly-rounded-divide-sqrt',
+]
+
+
+def torch_lib_dir() -> str:
+    import torch
+    return os.path.join(os.path.dirname(torch.__file__), 'lib')
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get('HIPCC'), '/opt/rocm/bin/hipcc', 'hipcc'):
+        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+            return cand
+    raise RuntimeError('hipcc not found')
+
+
+def build_library(verbose: bool = False) -> str:
+    """Compile every HIP source of the package into ``librr_hip.so``; returns its path."""
+    tl = torch_lib_dir()
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = OUT + '.tmp'
+    cmd = [hipcc()] + HIPCC_FLAGS + ['-o', tmp] + srcs + [f'-L{tl}', f'-Wl,-rpath,{tl}']
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    noise = "'-packed-fp32-ops' is not a recognized feature"
+    errs = '\n'.join(l for l in (res.stdout + res.stderr).splitlines() if noise not in l)
+    if res.returncode != 0:
+        raise RuntimeError(f'hipcc failed ({res.returncode}):\n{errs}')
+    if verbose and errs.strip():
+        print(errs, file=sys.stderr)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == '__main__':
+    print(build_library(verbose=True))

@@ -1,0 +1,232 @@
+// Device-side building blocks of the RoadRunner model for gfx950 (CDNA4).
+//
+// Geometry, limb-darkening interpolation, Kepler/Taylor orbit and the epoch
+// reduction, written once and instantiated for FP64 and FP32 where the flux
+// stage needs both.  Reference citations are to PyTransit 2.8.1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rr {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 2.0 * kPi;
+constexpr double kHalfPi = 0.5 * kPi;
+constexpr float kPiF = 3.14159265358979323846f;
+
+// ---- overloads so templates read the same for float and double -----------------------
+__device__ __forceinline__ double d_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float d_sqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double d_atan2(double y, double x) { return atan2(y, x); }
+__device__ __forceinline__ float d_atan2(float y, float x) { return atan2f(y, x); }
+__device__ __forceinline__ double d_abs(double x) { return fabs(x); }
+__device__ __forceinline__ float d_abs(float x) { return fabsf(x); }
+__device__ __forceinline__ double d_max(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float d_max(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double d_min(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float d_min(float a, float b) { return fminf(a, b); }
+template <typename T> __device__ __forceinline__ T pi_of();
+template <> __device__ __forceinline__ double pi_of<double>() { return kPi; }
+template <> __device__ __forceinline__ float pi_of<float>() { return kPiF; }
+
+// numpy.linspace element i of n: i*step + start with the last element exactly b.
+__device__ __forceinline__ double linspace_at(double a, double b, int n, int i) {
+#pragma clang fp contract(off)
+    if (i == n - 1) return b;
+    double step = (b - a) / (double)(n - 1);
+    return (double)i * step + a;
+}
+
+// Area of the intersection of two circles; common.py:36-49 (FP64, init-time tables).
+__device__ __forceinline__ double cci_area(double r1, double r2, double b) {
+    if (r1 < b - r2) return 0.0;
+    if (r1 >= b + r2) return kPi * (r2 * r2);
+    if (b - r2 <= -r1) return kPi * (r1 * r1);
+    return r2 * r2 * acos((b * b + r2 * r2 - r1 * r1) / (2.0 * b * r2)) +
+           r1 * r1 * acos((b * b + r1 * r1 - r2 * r2) / (2.0 * b * r1)) -
+           0.5 * sqrt((-b + r2 + r1) * (b + r2 - r1) * (b - r2 + r1) * (b + r2 + r1));
+}
+
+// Lens area and kappa_0 of circles r1, r2 at separation b by the kite formula
+// (Agol et al. 2020); common.py:52-73 with tsort common.py:5-33.
+template <typename T>
+__device__ __forceinline__ void ccia_kite(T r1, T r2, T b, T& area, T& kap0) {
+    if (r1 + r2 <= b) { area = T(0); kap0 = T(0); return; }
+    if (d_abs(r1 - r2) < b && b <= r1 + r2) {
+        // descending sort of (r1, r2, b): x >= y >= z
+        const T hi12 = d_max(r1, r2), lo12 = d_min(r1, r2);
+        const T x = d_max(hi12, b);
+        const T z = d_min(lo12, b);
+        const T y = d_max(lo12, d_min(hi12, b));
+        const T akite = T(0.5) * d_sqrt((x + (y + z)) * (z - (x - y)) * (z + (x - y)) * (x + (y - z)));
+        const T k0 = d_atan2(T(2) * akite, (r2 - r1) * (r2 + r1) + b * b);
+        const T k1 = d_atan2(T(2) * akite, (r1 - r2) * (r1 + r2) + b * b);
+        area = r1 * r1 * k1 + r2 * r2 * k0 - akite;
+        kap0 = k0;
+        return;
+    }
+    if (b <= r1 - r2) { area = pi_of<T>() * (r2 * r2); kap0 = pi_of<T>(); return; }
+    if (b <= r2 - r1) { area = pi_of<T>() * (r1 * r1); kap0 = T(0); return; }
+    area = T(NAN); kap0 = T(NAN);
+}
+
+// Mean limb darkening for grazing parameter g; common.py:225-233.  The interval
+// index is clamped to ng-2 (the reference reads lda[ng] for g in (1-1e-7, 1]).
+__device__ __forceinline__ double interp_ld(double g, double dg, const double* __restrict__ row, int ng) {
+    if (g < 0.0) return NAN;
+    if (g > 1.0) return 0.0;
+    int i = (int)floor(g / dg);
+    i = i > ng - 2 ? ng - 2 : (i < 0 ? 0 : i);
+    const double a = (g - i * dg) / dg;
+    return (1.0 - a) * row[i] + a * row[i + 1];
+}
+
+// FP32 form: g / dg by a multiply with 1/dg.  The lerp is continuous across
+// knots, so a knot assigned to the neighbouring interval changes the value by
+// rounding only.
+__device__ __forceinline__ float interp_ld_f(float g, float inv_dg, const float* __restrict__ row, int ng) {
+    if (g > 1.0f) return 0.0f;
+    const float x = g * inv_dg;
+    int i = (int)x;  // g >= 0
+    i = i > ng - 2 ? ng - 2 : (i < 0 ? 0 : i);
+    const float a = x - (float)i;
+    const float l0 = row[i], l1 = row[i + 1];
+    return fmaf(a, l1 - l0, l0);
+}
+
+// ---- Orbit: Kepler solve (orbits_py.py) and Taylor coefficients (taylor_z.py) ---------
+
+// numpy.mod: the result has the sign of the divisor.
+__device__ __forceinline__ double py_mod(double x, double y) {
+    double r = fmod(x, y);
+    if (r != 0.0) {
+        if ((r < 0.0) != (y < 0.0)) r += y;
+    } else {
+        r = copysign(0.0, y);
+    }
+    return r;
+}
+
+// orbits_py.py:82-86
+__device__ __forceinline__ double mean_anomaly_offset(double e, double w) {
+    double m = atan2(sqrt(1.0 - e * e) * sin(kHalfPi - w), e + cos(kHalfPi - w));
+    return m - e * sin(m);
+}
+
+// True anomaly at time t for t0 = 0: mean anomaly (orbits_py.py:115-119), Newton
+// eccentric anomaly with tol 1e-8 and at most 1000 steps (:144-154), true anomaly
+// (:191-196).  The Newton loop is bounded, so every lane exits.
+__device__ __forceinline__ double ta_newton(double t, double p, double e, double w) {
+    const double off = mean_anomaly_offset(e, w);
+    const double ma = py_mod(kTwoPi * (t - (0.0 - off * p / kTwoPi)) / p, kTwoPi);
+    double ea = ma, err = 0.05;
+    int k = 0;
+    while (fabs(err) > 1e-8 && k < 1000) {
+        err = ea - e * sin(ea) - ma;
+        ea = ea - err / (1.0 - e * cos(ea));
+        ++k;
+    }
+    const double ce = cos(ea), se = sin(ea);
+    const double sta = sqrt(1.0 - e * e) * se / (1.0 - e * ce);
+    const double cta = (ce - e) / (1.0 - e * ce);
+    return atan2(sta, cta);
+}
+
+// Sky-plane position at time t (t0 = 0) for the finite-difference stencil; taylor_z.py:54-76.
+__device__ __forceinline__ void kepler_xy(double t, double p, double a, double ci, double e, double w,
+                                          double& x, double& y) {
+    const double f = ta_newton(t, p, e, w);
+    const double r = a * (1.0 - e * e) / (1.0 + e * cos(f));
+    x = -r * cos(w + f);
+    y = -r * sin(w + f) * ci;
+}
+
+// Taylor coefficients from the seven stencil positions x[0..6], y[0..6] at
+// t = (j-3)*dt, dt = 0.02 d; taylor_z.py:78-102.  c[0..4] is x, c[5..9] is y, each
+// divided by n!; the x constant term is 0 (taylor_z.py:224).
+__device__ __forceinline__ void taylor_from_stencil(const double* x, const double* y, double* c) {
+    const double dt = 2e-2;
+    const double a1 = 1.0 / 60, b1 = 9.0 / 60, c1 = 45.0 / 60;
+    const double a2 = 1.0 / 90, b2 = 3.0 / 20, c2 = 3.0 / 2, d2 = 49.0 / 18;
+    const double a3 = 1.0 / 8, b3 = 1.0, c3 = 13.0 / 8;
+    const double a4 = 1.0 / 6, b4 = 2.0, c4 = 13.0 / 2, d4 = 28.0 / 3;
+    const double dt2 = dt * dt, dt3 = dt2 * dt, dt4 = dt3 * dt;
+    const double vx = (a1 * (x[6] - x[0]) + b1 * (x[1] - x[5]) + c1 * (x[4] - x[2])) / dt;
+    const double vy = (a1 * (y[6] - y[0]) + b1 * (y[1] - y[5]) + c1 * (y[4] - y[2])) / dt;
+    const double ax = (a2 * (x[0] + x[6]) - b2 * (x[1] + x[5]) + c2 * (x[2] + x[4]) - d2 * x[3]) / dt2;
+    const double ay = (a2 * (y[0] + y[6]) - b2 * (y[1] + y[5]) + c2 * (y[2] + y[4]) - d2 * y[3]) / dt2;
+    const double jx = (a3 * (x[0] - x[6]) + b3 * (x[5] - x[1]) + c3 * (x[2] - x[4])) / dt3;
+    const double jy = (a3 * (y[0] - y[6]) + b3 * (y[5] - y[1]) + c3 * (y[2] - y[4])) / dt3;
+    const double sx = (-a4 * (x[0] + x[6]) + b4 * (x[1] + x[5]) - c4 * (x[2] + x[4]) + d4 * x[3]) / dt4;
+    const double sy = (-a4 * (y[0] + y[6]) + b4 * (y[1] + y[5]) - c4 * (y[2] + y[4]) + d4 * y[3]) / dt4;
+    c[0] = 0.0; c[1] = vx; c[2] = 0.5 * ax; c[3] = jx / 6.0; c[4] = sx / 24.0;
+    c[5] = y[3]; c[6] = vy; c[7] = 0.5 * ay; c[8] = jy / 6.0; c[9] = sy / 24.0;
+}
+
+// Separation z(t) from the Taylor coefficients by Horner's rule (sep_c).
+template <typename T>
+__device__ __forceinline__ T taylor_z(T t, const T* c) {
+    const T px = (((c[4] * t + c[3]) * t + c[2]) * t + c[1]) * t + c[0];
+    const T py = (((c[9] * t + c[8]) * t + c[7]) * t + c[6]) * t + c[5];
+    return d_sqrt(px * px + py * py);
+}
+
+// z_taylor_s(t, t0=0, p=1): Taylor z with a unit-period phase wrap; taylor_z.py:198-226.
+__device__ __forceinline__ double taylor_z_wrapped(double t, const double* c) {
+    const double epoch = floor((t - 0.0 + 0.5 * 1.0) / 1.0);
+    const double tt = t - (0.0 + epoch * 1.0);
+    return taylor_z(tt, c);
+}
+
+// Contact time by bisection, tol 1e-6 d and at most 100 steps; taylor_z.py:298-328.
+// point 1 or 4 (first/fourth contact, z = 1+k).
+__device__ __forceinline__ double find_contact_point(double k, int point, const double* c) {
+    const double s = (point == 1 || point == 2 || point == 12) ? -1.0 : 1.0;
+    const double zt = (point == 1 || point == 4) ? 1.0 + k : ((point == 2 || point == 3) ? 1.0 - k : 1.0);
+    double t0 = 0.0;
+    double t2 = s * 2.0 / c[1];
+    double t1 = 0.5 * t2;
+    double z0 = taylor_z_wrapped(t0, c) - zt;
+    double z1 = taylor_z_wrapped(t1, c) - zt;
+    int i = 0;
+    while (fabs(t2 - t0) > 1e-6 && i < 100) {
+        if (z0 * z1 < 0.0) {
+            t2 = t1;
+            t1 = 0.5 * (t0 + t1);
+            z1 = taylor_z_wrapped(t1, c) - zt;
+        } else {
+            t0 = t1;
+            t1 = 0.5 * (t1 + t2);
+            z0 = z1;
+            z1 = taylor_z_wrapped(t1, c) - zt;
+        }
+        ++i;
+    }
+    return t1;
+}
+
+// ---- Epoch reduction (model_full.py:88-89) -------------------------------------------
+
+// Exact form: floor((t - t0 + p/2) / p) with an IEEE division, then
+// tc = t - (t0 + epoch*p), with no contraction so each step rounds as in numpy.
+__device__ __forceinline__ double phase_exact(double t, double t0, double p) {
+#pragma clang fp contract(off)
+    const double epoch = floor((t - t0 + 0.5 * p) / p);
+    return t - (t0 + epoch * p);
+}
+
+// Division-free form for the mixed path: epoch from a multiply by 1/p, then one
+// correction step so that r = (t - t0 + p/2) - epoch*p lies in [0, p).  It can
+// differ from phase_exact only where the quotient is within rounding of an
+// integer, i.e. at tc = +-p/2, far outside any transit window.
+__device__ __forceinline__ double phase_fast(double t, double t0, double p, double inv_p) {
+#pragma clang fp contract(off)
+    const double u = t - t0 + 0.5 * p;
+    double epoch = floor(u * inv_p);
+    const double r = u - epoch * p;
+    epoch += (r < 0.0) ? -1.0 : ((r >= p) ? 1.0 : 0.0);
+    return t - (t0 + epoch * p);
+}
+
+}  // namespace rr

@@ -1,0 +1,30 @@
+// Types and helpers shared by the library's translation units (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+
+namespace rr {
+
+// Per-parameter-vector state written by the setup kernels and read by the flux
+// kernels: Taylor coefficients (x then y, divided by n!), the contact window
+// (first/fourth contact, before the per-light-curve margin), period, 1/period,
+// the radius ratio the window and the k-table lerp used, and a validity flag.
+struct alignas(16) PvRec {
+    double c[10];
+    double bt1, bt4;
+    double p, inv_p;
+    double k0;
+    int32_t good;
+    int32_t pad;
+};
+static_assert(sizeof(PvRec) == 128, "PvRec must be 128 bytes");
+
+int set_error(int code, const char* fmt, ...);
+int check_launch(const char* what);
+size_t align_up(size_t x, size_t a);
+
+}  // namespace rr
