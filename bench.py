@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark of the RoadRunner hot path on one MI355X.
+
+A step is one ``RoadRunnerModel.evaluate`` of the whole parameter population on
+device-resident inputs: limb-darkening profiles, per-pv setup (Taylor orbit,
+contact window, limb-darkening means) and the flux kernel.  The workload is the
+BJD-scale population: 4,096 parameter vectors x 20,000 times (2-min cadence from
+BJD 2459000.0), 10 supersamples over a 30-min exposure, power-2 limb darkening.
+
+``value`` is model points/s (one point = one output flux value) of the mixed
+precision path (FP64 epoch reduction and setup, FP32 flux).  The same workload is
+timed in plain FP64 and the maximum |F_mixed - F_fp64| over every point is
+reported in ppm.  ``roofline`` is for the flux kernel alone, timed with HIP events
+on the stream it is launched on; ``cpu_baseline`` is the C/OpenMP FP64
+restatement of the reference algorithm on this host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+METRIC = "model points/s mixed vs FP64 plain at ≤1 ppm flux err; served p50/p99 lat"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--cpu-reps', type=int, default=5)
+    return ap.parse_args()
+
+
+def flux_kernel_bytes(npv, npt, ng, out_bytes):
+    """Algorithmic HBM bytes of one flux launch: the flux write, one read of the
+    times, and per pv the setup record (128 B), its limb-darkening-mean row
+    (ng values) and k, t0, istar (8 B each)."""
+    row = ng * (4 if out_bytes == 4 else 8)
+    return npv * npt * out_bytes + npt * 8 + npv * (128 + row + 24)
+
+
+def time_path(tm, args_dev, steps, warmup):
+    import torch
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for _ in range(warmup):
+        tm._evaluate_device(*args_dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for s in range(steps):
+        out = tm._evaluate_device(*args_dev, flux_events=ev[s])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = statistics.mean(a.elapsed_time(b) for a, b in ev)
+    return wall / steps, kern_ms, out
+
+
+def cpu_baseline(wl, reps):
+    from oracle import rr_cpu
+    ncores = len(os.sched_getaffinity(0))
+    nthreads = min(ncores, int(os.environ.get('OMP_NUM_THREADS', ncores)))
+    cm = rr_cpu.CpuRR('power-2')
+    cm.set_data(wl.times, nsamples=wl.nsamples, exptimes=wl.exptimes)
+    args = (wl.k, wl.ldc, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w)
+    cm.evaluate(*args, nthreads=nthreads)
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        cm.evaluate(*args, nthreads=nthreads)
+        ts.append(time.perf_counter() - t)
+    med = statistics.median(ts)
+    npts = wl.k.shape[0] * wl.times.size
+    # single-thread figure on a 256-pv slice of the same workload
+    n1 = 256
+    a1 = tuple(x[:n1] for x in args)
+    t = time.perf_counter()
+    cm.evaluate(*a1, nthreads=1)
+    t1 = time.perf_counter() - t
+    cpu_model = ''
+    try:
+        with open('/proc/cpuinfo') as f:
+            cpu_model = next(l.split(':', 1)[1].strip() for l in f if l.startswith('model name'))
+    except (OSError, StopIteration):
+        pass
+    return {"value": npts / med, "unit": "points/s", "cores": nthreads, "kind": "port",
+            "sample": f"full workload ({wl.k.shape[0]}x{wl.times.size} points, nss=10, power-2), C/OpenMP FP64 "
+                      f"restatement of rr_full, median of {reps} runs after 1 warm-up, LD evaluation included",
+            "value_1thread": n1 * wl.times.size / t1, "sample_1thread": f"{n1}x{wl.times.size} points, 1 thread",
+            "cpu_model": cpu_model}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path has no multi-GPU form "
+              f"in this build.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from pytransit_amd import RoadRunnerModel, workloads
+
+    dev = torch.device('cuda', 0)
+    wl = workloads.rr_bjd()
+    npv, npt = wl.k.shape[0], wl.times.size
+    args_dev = tuple(torch.as_tensor(x, device=dev) for x in (wl.k, wl.ldc, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w))
+
+    res = {}
+    for prec in ('mixed', 'fp64'):
+        tm = RoadRunnerModel(wl.law, precision=prec, device=dev)
+        tm.set_data(wl.times, nsamples=wl.nsamples, exptimes=wl.exptimes)
+        step_s, kern_ms, out = time_path(tm, args_dev, args.steps, args.warmup)
+        res[prec] = dict(step_s=step_s, kern_ms=kern_ms, out=out, ng=tm.ng)
+
+    err = (res['mixed']['out'].double() - res['fp64']['out']).abs()
+    max_err_ppm = float(err.max()) * 1e6
+    both_nan = bool(torch.equal(torch.isnan(res['mixed']['out']), torch.isnan(res['fp64']['out'])))
+
+    npts = npv * npt
+    mixed_pps = npts / res['mixed']['step_s']
+    fp64_pps = npts / res['fp64']['step_s']
+    kbytes = flux_kernel_bytes(npv, npt, res['mixed']['ng'], 4)
+    achieved = kbytes / (res['mixed']['kern_ms'] * 1e-3) / 1e9
+    traffic = None
+    line = {
+        "metric": METRIC,
+        "value": mixed_pps,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res['mixed']['step_s'] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"rr_bjd: RoadRunnerModel('power-2') evaluate, npv={npv} x npt={npt} "
+                               f"(2-min cadence from BJD 2459000.0), nss=10 over 30 min, mixed precision",
+                   "npv": npv, "npt": npt, "nsamples": 10, "ld": "power-2"},
+        "fp64_points_per_s": fp64_pps,
+        "fp64_ms_per_step": res['fp64']['step_s'] * 1e3,
+        "mixed_over_fp64": mixed_pps / fp64_pps,
+        "max_abs_err_ppm_mixed_vs_fp64": max_err_ppm,
+        "nan_pattern_equal": both_nan,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "rr_flux_kernel<mixed>", "kernel_ms": res['mixed']['kern_ms'],
+                     "algorithmic_bytes": kbytes,
+                     "fp64_kernel_ms": res['fp64']['kern_ms']},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(wl, args.cpu_reps)
+        line["cpu_baseline"] = cb
+        line["speedup_vs_cpu"] = mixed_pps / cb["value"]
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

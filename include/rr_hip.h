@@ -123,6 +123,16 @@ int rr_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
                 const double* ldp, const double* istar, void* workspace, size_t workspace_bytes,
                 void* flux, void* stream);
 
+/* The two stages of rr_evaluate, for callers that time or reuse them separately.
+ * rr_setup writes the per-pv state (validity, Taylor orbit, contact window,
+ * limb-darkening means; model_full.py:39-70) into the workspace; rr_flux reads it
+ * and writes the flux (the hot loop, model_full.py:75-99).  rr_flux must follow
+ * an rr_setup with the same precision, sizes and workspace. */
+int rr_setup(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
+             const double* ldp, void* workspace, size_t workspace_bytes, void* stream);
+int rr_flux(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
+            const double* istar, void* workspace, size_t workspace_bytes, void* flux, void* stream);
+
 /* Orbit boundary alone: Taylor coefficients coeffs[npv, 2, 5] (row 0 x, row 1 y,
  * coefficient n = d^n/dt^n / n!) and contact window win[npv, 2] = (t1, t4) for
  * radius ratio k0[npv].  Replaces meepmeep's solve2d + bounding_box as called at

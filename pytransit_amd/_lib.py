@@ -57,6 +57,10 @@ _SIGNATURES = {
     'rr_integrate_profiles': (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     'rr_orbit': (c_int, [c_int] + [c_void_p] * 9),
     'rr_workspace_bytes': (c_size_t, [c_int, c_int, c_int, c_int]),
+    'rr_setup': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), c_void_p, c_void_p,
+                         c_size_t, c_void_p]),
+    'rr_flux': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), c_void_p, c_void_p,
+                        c_size_t, c_void_p, c_void_p]),
     'rr_evaluate': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), c_void_p, c_void_p,
                             c_void_p, c_size_t, c_void_p, c_void_p]),
 }

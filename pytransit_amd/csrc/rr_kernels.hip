@@ -482,58 +482,85 @@ size_t rr_workspace_bytes(int npv, int npb, int ng, int nlc) {
     return b;
 }
 
-int rr_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const double* ldp,
-                const double* istar, void* workspace, size_t workspace_bytes, void* flux, void* stream) {
-    if (!tab || !lc || !pv) return set_error(RR_EINVAL, "rr_evaluate: null descriptor");
-    if (precision != RR_FP64 && precision != RR_MIXED) return set_error(RR_EINVAL, "rr_evaluate: bad precision");
+static int validate(const char* who, int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
+                    size_t workspace_bytes, const void* workspace) {
+    if (!tab || !lc || !pv) return set_error(RR_EINVAL, "%s: null descriptor", who);
+    if (precision != RR_FP64 && precision != RR_MIXED) return set_error(RR_EINVAL, "%s: bad precision", who);
     const int npv = pv->npv, npt = lc->npt, npb = lc->npb, nlc = lc->nlc;
     if (npv < 0 || npt < 0 || nlc < 1 || npb < 1 || lc->nep < 1)
-        return set_error(RR_EINVAL, "rr_evaluate: bad sizes (npv=%d npt=%d nlc=%d npb=%d nep=%d)", npv, npt, nlc,
-                         npb, lc->nep);
+        return set_error(RR_EINVAL, "%s: bad sizes (npv=%d npt=%d nlc=%d npb=%d nep=%d)", who, npv, npt, nlc, npb,
+                         lc->nep);
     if (pv->kcols != 1 && pv->kcols != npb)
         return set_error(RR_EINVAL, "Radius ratios should be given either as an [npv, 1] or [npv, npb] array.");
-    if (pv->t0cols < 1) return set_error(RR_EINVAL, "rr_evaluate: t0 needs at least one column");
+    if (pv->t0cols < 1) return set_error(RR_EINVAL, "%s: t0 needs at least one column", who);
     if (tab->nk < 2 || tab->ng < 2 || tab->nz < 1 || !tab->weights || !tab->ze)
-        return set_error(RR_EINVAL, "rr_evaluate: bad weight table");
-    if (npv == 0 || npt == 0) return RR_OK;
-    if (!lc->times || !lc->pbids || !lc->epids || !lc->nsamples || !lc->exptimes || (nlc > 1 && !lc->lcids))
-        return set_error(RR_EINVAL, "rr_evaluate: null light-curve array");
-    if (!pv->k || !pv->t0 || !pv->p || !pv->a || !pv->inc || !pv->e || !pv->w || !ldp || !istar || !flux)
-        return set_error(RR_EINVAL, "rr_evaluate: null parameter array");
+        return set_error(RR_EINVAL, "%s: bad weight table", who);
     if ((long long)npv * npt > (1LL << 40) || (long long)npv * npb * tab->ng > (1LL << 31))
-        return set_error(RR_ERANGE, "rr_evaluate: problem too large");
+        return set_error(RR_ERANGE, "%s: problem too large", who);
+    if (npv == 0 || npt == 0) return RR_OK;
     const size_t need = rr_workspace_bytes(npv, npb, tab->ng, nlc);
     if (!workspace || workspace_bytes < need)
-        return set_error(RR_EINVAL, "rr_evaluate: workspace of %zu bytes needed, got %zu", need, workspace_bytes);
+        return set_error(RR_EINVAL, "%s: workspace of %zu bytes needed, got %zu", who, need, workspace_bytes);
+    return RR_OK;
+}
 
+struct WsView {
+    PvRec* rec;
+    double* ldm64;
+    float* ldm32;
+};
+
+static WsView carve(void* workspace, int npv, int npb, int ng) {
     char* ws = (char*)workspace;
-    PvRec* rec = (PvRec*)ws;
+    WsView v;
+    v.rec = (PvRec*)ws;
     ws += align_up((size_t)npv * sizeof(PvRec), 256);
-    double* ldm64 = (double*)ws;
-    ws += align_up((size_t)npv * npb * tab->ng * sizeof(double), 256);
-    float* ldm32 = (float*)ws;
-    hipStream_t s = (hipStream_t)stream;
+    v.ldm64 = (double*)ws;
+    ws += align_up((size_t)npv * npb * ng * sizeof(double), 256);
+    v.ldm32 = (float*)ws;
+    return v;
+}
 
+int rr_setup(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const double* ldp,
+             void* workspace, size_t workspace_bytes, void* stream) {
+    int rc = validate("rr_setup", precision, tab, lc, pv, workspace_bytes, workspace);
+    if (rc) return rc;
+    const int npv = pv->npv, npb = lc->npb;
+    if (npv == 0 || lc->npt == 0) return RR_OK;
+    if (!pv->k || !pv->p || !pv->a || !pv->inc || !pv->e || !pv->w || !ldp)
+        return set_error(RR_EINVAL, "rr_setup: null parameter array");
+    WsView v = carve(workspace, npv, npb, tab->ng);
     SetupArgs sa;
     sa.k = pv->k; sa.p = pv->p; sa.a = pv->a; sa.inc = pv->inc; sa.e = pv->e; sa.w = pv->w;
     sa.ldp = ldp; sa.W = tab->weights; sa.ze = tab->ze;
-    sa.rec = rec; sa.ldm64 = ldm64; sa.ldm32 = precision == RR_MIXED ? ldm32 : nullptr;
+    sa.rec = v.rec; sa.ldm64 = v.ldm64; sa.ldm32 = precision == RR_MIXED ? v.ldm32 : nullptr;
     sa.npb = npb; sa.kcols = pv->kcols; sa.nk = tab->nk; sa.ng = tab->ng; sa.nz = tab->nz;
     sa.kmin = tab->kmin; sa.kmax = tab->kmax; sa.dk = tab->dk;
-    hipLaunchKernelGGL(rr_setup_kernel, dim3(npv), dim3(64), 0, s, sa);
-    int rc = check_launch("rr_setup_kernel");
-    if (rc) return rc;
+    hipLaunchKernelGGL(rr_setup_kernel, dim3(npv), dim3(64), 0, (hipStream_t)stream, sa);
+    return check_launch("rr_setup_kernel");
+}
 
+int rr_flux(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const double* istar,
+            void* workspace, size_t workspace_bytes, void* flux, void* stream) {
+    int rc = validate("rr_flux", precision, tab, lc, pv, workspace_bytes, workspace);
+    if (rc) return rc;
+    const int npv = pv->npv, npt = lc->npt, npb = lc->npb, nlc = lc->nlc;
+    if (npv == 0 || npt == 0) return RR_OK;
+    if (!lc->times || !lc->pbids || !lc->epids || !lc->nsamples || !lc->exptimes || (nlc > 1 && !lc->lcids))
+        return set_error(RR_EINVAL, "rr_flux: null light-curve array");
+    if (!pv->k || !pv->t0 || !istar || !flux) return set_error(RR_EINVAL, "rr_flux: null parameter array");
+    WsView v = carve(workspace, npv, npb, tab->ng);
     FluxArgs fa;
     fa.times = lc->times; fa.lcids = lc->lcids; fa.pbids = lc->pbids; fa.epids = lc->epids;
     fa.nsamples = lc->nsamples; fa.exptimes = lc->exptimes;
     fa.k = pv->k; fa.t0 = pv->t0; fa.istar = istar;
-    fa.rec = rec; fa.ldm64 = ldm64; fa.ldm32 = ldm32; fa.flux = flux;
+    fa.rec = v.rec; fa.ldm64 = v.ldm64; fa.ldm32 = v.ldm32; fa.flux = flux;
     fa.npt = npt; fa.nlc = nlc; fa.npb = npb; fa.nep = lc->nep; fa.kcols = pv->kcols; fa.t0cols = pv->t0cols;
     fa.ng = tab->ng; fa.dg = tab->dg;
     fa.ntiles = (npt + kFluxTile - 1) / kFluxTile;
     const long long nblocks = (long long)fa.ntiles * npv;
-    if (nblocks > 0x7fffffffLL) return set_error(RR_ERANGE, "rr_evaluate: grid too large");
+    if (nblocks > 0x7fffffffLL) return set_error(RR_ERANGE, "rr_flux: grid too large");
+    hipStream_t s = (hipStream_t)stream;
     const bool one = (nlc == 1);
     if (precision == RR_MIXED) {
         if (one) hipLaunchKernelGGL((rr_flux_kernel<true, true>), dim3((unsigned)nblocks), dim3(kFluxThreads), 0, s, fa);
@@ -543,6 +570,13 @@ int rr_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
         else hipLaunchKernelGGL((rr_flux_kernel<false, false>), dim3((unsigned)nblocks), dim3(kFluxThreads), 0, s, fa);
     }
     return check_launch("rr_flux_kernel");
+}
+
+int rr_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const double* ldp,
+                const double* istar, void* workspace, size_t workspace_bytes, void* flux, void* stream) {
+    int rc = rr_setup(precision, tab, lc, pv, ldp, workspace, workspace_bytes, stream);
+    if (rc) return rc;
+    return rr_flux(precision, tab, lc, pv, istar, workspace, workspace_bytes, flux, stream);
 }
 
 }  // extern "C"

@@ -290,6 +290,13 @@ class RoadRunnerModel(TransitModel):
         'mixed'.  Invalid parameter vectors (a <= 1, a NaN, e < 0 or NaN
         limb darkening) give NaN rows (model_full.py:40-42,80-82).
         """
+        flux = self._evaluate_device(k, ldc, t0, p, a, i, e, w).squeeze()
+        return flux.cpu().numpy() if copy else flux
+
+    def _evaluate_device(self, k, ldc, t0, p, a, i, e, w, flux_events=None):
+        """All three stages on torch's current stream; returns the [npv, npt] device
+        flux.  ``flux_events`` = (start, end) torch.cuda.Event pair recorded around
+        the flux kernel alone."""
         lib = _lib.load()
         device = self.device
         if self.time is None:
@@ -298,8 +305,6 @@ class RoadRunnerModel(TransitModel):
         npb = self.npb
         if k.shape[1] > 1 and k.shape[1] != npb:
             raise ValueError('Radius ratios should be given either as an [npv, 1] or [npv, npb] array.')
-        if k.shape[1] != 1 and k.shape[1] != npb:
-            k = k[:, :npb].contiguous()
         if self.epids.max() >= t0.shape[1]:
             raise ValueError(f"t0 has {t0.shape[1]} columns but the epoch indices go up to {self.epids.max()}")
         if not isscalar(ldc) and not isinstance(ldc, torch.Tensor):
@@ -317,10 +322,16 @@ class RoadRunnerModel(TransitModel):
         flux = torch.empty((npv, self.npt), dtype=dtype, device=device)
         nbytes = lib.rr_workspace_bytes(npv, npb, self.ng, self.nlc)
         ws = self._workspace(device, nbytes)
-        check(lib.rr_evaluate(prec, _lib.ctypes.byref(tab), _lib.ctypes.byref(lc), _lib.ctypes.byref(prm),
-                              ptr(ldp), ptr(istar), ptr(ws), ws.numel(), ptr(flux), stream_handle(device)))
-        flux = flux.squeeze()
-        return flux.cpu().numpy() if copy else flux
+        stream = stream_handle(device)
+        byref = _lib.ctypes.byref
+        check(lib.rr_setup(prec, byref(tab), byref(lc), byref(prm), ptr(ldp), ptr(ws), ws.numel(), stream))
+        if flux_events is not None:
+            flux_events[0].record()
+        check(lib.rr_flux(prec, byref(tab), byref(lc), byref(prm), ptr(istar), ptr(ws), ws.numel(), ptr(flux),
+                          stream))
+        if flux_events is not None:
+            flux_events[1].record()
+        return flux
 
     def __call__(self, *args, **kwargs):
         return self.evaluate(*args, **kwargs)
