@@ -46,20 +46,39 @@ extern "C" {
 #define RR_LD_POWER_2 9
 #define RR_LD_POWER_2_PM 10
 
-/* Limb-darkening weight table (common.py:188-223), device-resident. */
+/* Limb-darkening weight table (common.py:188-223), device-resident.  The device
+ * layout is [nk, nz, ng] (g fastest, so lanes indexed by g read it coalesced);
+ * the reference's array is weights[ik, ig, iz]. */
 typedef struct rr_table {
     int32_t nk, ng, nz;
     int32_t pad_;
     double kmin, kmax;
     double dk;               /* (kmax-kmin)/nk, the reference's indexing step (Q1)   */
     double dg;               /* gs[1]-gs[0] of linspace(0, 1-1e-7, ng)               */
-    const double* weights;   /* [nk, ng, nz] FP64; may be NULL for TS exact weights  */
+    const double* weights;   /* [nk, nz, ng] FP64; NULL selects exact 2-D weights    */
+    const float* weights32;  /* [nk, nz, ng] FP32 copy used by RR_MIXED; nullable    */
     const double* ze;        /* [nz] annulus edges                                   */
+    const double* mu;        /* [nz] mu at the annulus means (rrmodel.py:172)        */
 } rr_table;
+
+/* Limb darkening of one evaluate() call.  law >= 0: a named law (RR_LD_*) with
+ * coefficients ldc[npv, npb, nldc], evaluated on the device at tab->mu.
+ * law == RR_LD_TABULATED: profiles ldp[npv, npb, nz] and integrals istar[npv, npb]
+ * given by the caller (the LDModel protocol, ldmodel.py:21-39). */
+#define RR_LD_TABULATED (-1)
+typedef struct rr_ld {
+    int32_t law;
+    int32_t nldc;
+    const double* ldc;
+    const double* ldp;
+    const double* istar;
+} rr_ld;
 
 /* Light-curve data set by set_data() (transitmodel.py:56-125), device-resident. */
 typedef struct rr_lcdata {
     int32_t npt, nlc, npb, nep;
+    int32_t sorted;           /* 1 if times are non-decreasing (enables run skipping) */
+    int32_t uniform_nsamples; /* the common nsamples of all light curves, or 0         */
     const double* times;      /* [npt]                                               */
     const int32_t* lcids;     /* [npt], values in [0, nlc)                           */
     const int32_t* pbids;     /* [nlc], values in [0, npb)                           */
@@ -90,8 +109,9 @@ int rr_abi_version(void);
  * on the host.  Replaces create_z_grid (common.py:131-149) and rrmodel.py:171-172. */
 int rr_z_grid(double zcut, int nin, int nedge, double* ze, double* zm, double* mu);
 
-/* 3-D weight table W[nk, ng, nz] (FP64, device) on linspace(kmin, kmax, nk) x
- * linspace(0, 1-1e-7, ng); optional FP32 copy W32 (NULL to skip).  Writes the
+/* 3-D weight table (device) on linspace(kmin, kmax, nk) x linspace(0, 1-1e-7, ng),
+ * written in the device layout W[nk, nz, ng] (FP64) with an optional FP32 copy
+ * W32 (NULL to skip).  Writes the
  * reference's dk = (kmax-kmin)/nk and dg to *dk_out / *dg_out (host pointers).
  * Replaces calculate_weights_3d (common.py:188-223), called at rrmodel.py:173. */
 int rr_weights_3d(int nk, double kmin, double kmax, const double* ze, int nz, int ng,
@@ -110,28 +130,28 @@ int rr_ld_profiles(int law, const double* ldc, int npv, int npb, int nldc,
 int rr_integrate_profiles(const double* mu, int nmu, const double* ldp, int npv, int npb,
                           double* istar, void* stream);
 
-/* Bytes of device workspace rr_evaluate / ts_evaluate need for these sizes. */
-size_t rr_workspace_bytes(int npv, int npb, int ng, int nlc);
+/* Bytes of device workspace rr_evaluate / rr_setup / rr_flux need for these sizes:
+ * per-pv state, limb-darkening means and the queue of in-window points. */
+size_t rr_workspace_bytes(int npv, int npt, int npb, int ng, int nlc);
 
-/* The full RoadRunner model: per-pv setup (validity, Taylor orbit, contact-time
- * window, limb-darkening means) then the flux over all (pv, time) points.
- * flux is [npv, npt]: double* for RR_FP64, float* for RR_MIXED.
- * ldp is [npv, npb, nz] and istar [npv, npb] (FP64).
+/* The full RoadRunner model: per-pv setup (validity, limb-darkening profiles and
+ * means, Taylor orbit, contact-time window) then the flux over all (pv, time)
+ * points.  flux is [npv, npt]: double* for RR_FP64, float* for RR_MIXED.
  * Replaces rr_full (pytransit/models/roadrunner/model_full.py:9-100) and its
- * npv=1 specialisation rr_simple (model_simple.py:11-138). */
+ * npv=1 specialisation rr_simple (model_simple.py:11-138), together with the
+ * limb-darkening evaluation of RoadRunnerModel.evaluate (rrmodel.py:215-227). */
 int rr_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
-                const double* ldp, const double* istar, void* workspace, size_t workspace_bytes,
-                void* flux, void* stream);
+                const rr_ld* ld, void* workspace, size_t workspace_bytes, void* flux, void* stream);
 
 /* The two stages of rr_evaluate, for callers that time or reuse them separately.
- * rr_setup writes the per-pv state (validity, Taylor orbit, contact window,
- * limb-darkening means; model_full.py:39-70) into the workspace; rr_flux reads it
- * and writes the flux (the hot loop, model_full.py:75-99).  rr_flux must follow
- * an rr_setup with the same precision, sizes and workspace. */
+ * rr_setup writes the per-pv state (model_full.py:39-70 plus the profiles) into
+ * the workspace; rr_flux reads it and writes the flux (the hot loop,
+ * model_full.py:75-99).  rr_flux must follow an rr_setup with the same
+ * precision, sizes and workspace. */
 int rr_setup(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
-             const double* ldp, void* workspace, size_t workspace_bytes, void* stream);
+             const rr_ld* ld, void* workspace, size_t workspace_bytes, void* stream);
 int rr_flux(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
-            const double* istar, void* workspace, size_t workspace_bytes, void* flux, void* stream);
+            void* workspace, size_t workspace_bytes, void* flux, void* stream);
 
 /* Orbit boundary alone: Taylor coefficients coeffs[npv, 2, 5] (row 0 x, row 1 y,
  * coefficient n = d^n/dt^n / n!) and contact window win[npv, 2] = (t1, t4) for

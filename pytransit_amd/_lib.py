@@ -28,14 +28,22 @@ LD_LAW_IDS = {
 }
 
 
+RR_LD_TABULATED = -1
+
+
 class RRTable(ctypes.Structure):
     _fields_ = [('nk', c_int32), ('ng', c_int32), ('nz', c_int32), ('pad_', c_int32),
                 ('kmin', c_double), ('kmax', c_double), ('dk', c_double), ('dg', c_double),
-                ('weights', c_void_p), ('ze', c_void_p)]
+                ('weights', c_void_p), ('weights32', c_void_p), ('ze', c_void_p), ('mu', c_void_p)]
+
+
+class RRLd(ctypes.Structure):
+    _fields_ = [('law', c_int32), ('nldc', c_int32), ('ldc', c_void_p), ('ldp', c_void_p), ('istar', c_void_p)]
 
 
 class RRLcData(ctypes.Structure):
     _fields_ = [('npt', c_int32), ('nlc', c_int32), ('npb', c_int32), ('nep', c_int32),
+                ('sorted', c_int32), ('uniform_nsamples', c_int32),
                 ('times', c_void_p), ('lcids', c_void_p), ('pbids', c_void_p), ('epids', c_void_p),
                 ('nsamples', c_void_p), ('exptimes', c_void_p)]
 
@@ -56,12 +64,12 @@ _SIGNATURES = {
     'rr_ld_profiles': (c_int, [c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     'rr_integrate_profiles': (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     'rr_orbit': (c_int, [c_int] + [c_void_p] * 9),
-    'rr_workspace_bytes': (c_size_t, [c_int, c_int, c_int, c_int]),
-    'rr_setup': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), c_void_p, c_void_p,
+    'rr_workspace_bytes': (c_size_t, [c_int, c_int, c_int, c_int, c_int]),
+    'rr_setup': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd), c_void_p,
                          c_size_t, c_void_p]),
-    'rr_flux': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), c_void_p, c_void_p,
-                        c_size_t, c_void_p, c_void_p]),
-    'rr_evaluate': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), c_void_p, c_void_p,
+    'rr_flux': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), c_void_p, c_size_t,
+                        c_void_p, c_void_p]),
+    'rr_evaluate': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd),
                             c_void_p, c_size_t, c_void_p, c_void_p]),
 }
 

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/rr_hip.h"
+
 namespace rr {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -227,6 +229,71 @@ __device__ __forceinline__ double phase_fast(double t, double t0, double p, doub
     const double r = u - epoch * p;
     epoch += (r < 0.0) ? -1.0 : ((r >= p) ? 1.0 : 0.0);
     return t - (t0 + epoch * p);
+}
+
+// ---- Limb-darkening laws; models/numba/ldmodels.py:22-139 -----------------------------
+
+__device__ __forceinline__ double ld_law(int law, double mu, const double* c, int nc) {
+    switch (law) {
+        case RR_LD_UNIFORM: return 1.0;
+        case RR_LD_LINEAR: return 1.0 - c[0] * (1.0 - mu);
+        case RR_LD_QUADRATIC: return 1.0 - c[0] * (1.0 - mu) - c[1] * ((1.0 - mu) * (1.0 - mu));
+        case RR_LD_QUADRATIC_TRI: {
+            const double a = sqrt(c[0]), b = 2.0 * c[1];
+            const double u = a * b, v = a * (1.0 - b);
+            return 1.0 - u * (1.0 - mu) - v * ((1.0 - mu) * (1.0 - mu));
+        }
+        case RR_LD_NONLINEAR:
+            return 1.0 - c[0] * (1.0 - sqrt(mu)) - c[1] * (1.0 - mu) - c[2] * (1.0 - pow(mu, 1.5)) -
+                   c[3] * (1.0 - mu * mu);
+        case RR_LD_GENERAL: {
+            double s = 0.0;
+            for (int i = 0; i < nc; ++i) s += c[i] * (1.0 - pow(mu, (double)(i + 1)));
+            return s;
+        }
+        case RR_LD_SQUARE_ROOT: return 1.0 - c[0] * (1.0 - mu) - c[1] * (1.0 - sqrt(mu));
+        case RR_LD_LOGARITHMIC: return 1.0 - c[0] * (1.0 - mu) - c[1] * mu * log(mu);
+        case RR_LD_EXPONENTIAL: return 1.0 - c[0] * (1.0 - mu) - c[1] / (1.0 - exp(mu));
+        case RR_LD_POWER_2: return 1.0 - c[0] * (1.0 - pow(mu, c[1]));
+        case RR_LD_POWER_2_PM: {
+            const double cc = 1.0 - c[0] + c[1];
+            const double a = log2(cc / c[1]);
+            return 1.0 - cc * (1.0 - pow(mu, a));
+        }
+        default: return NAN;
+    }
+}
+
+// Analytic disk integral where the reference has one (ldmodels.py:27-123); false
+// for the laws that use the 200-node trapezoid (rrmodel.py:220-227, Q8).
+__device__ __forceinline__ bool ld_integral(int law, const double* c, double& out) {
+    switch (law) {
+        case RR_LD_UNIFORM: out = kPi; return true;
+        case RR_LD_LINEAR: out = 2.0 * kPi * 1.0 / 6.0 * (3.0 - 2.0 * c[0]); return true;  // Q7, as the reference
+        case RR_LD_QUADRATIC: out = 2.0 * kPi * 1.0 / 12.0 * (-2.0 * c[0] - c[1] + 6.0); return true;
+        case RR_LD_QUADRATIC_TRI: {
+            const double a = sqrt(c[0]), b = 2.0 * c[1];
+            const double u = a * b, v = a * (1.0 - b);
+            out = 2.0 * kPi * 1.0 / 12.0 * (-2.0 * u - v + 6.0);
+            return true;
+        }
+        case RR_LD_POWER_2: out = 2.0 * kPi * (-c[0] * c[1] + c[1] + 2.0) / (2.0 * c[1] + 4.0); return true;
+        default: return false;
+    }
+}
+
+// One trapezoid segment m (1..199) of 2*pi*trapz(z*I(mu), z) on mu = linspace(1, 0, 200).
+__device__ __forceinline__ double ld_trapz_segment(int law, const double* c, int nc, int m) {
+    const double mu1 = linspace_at(1.0, 0.0, 200, m), mu0 = linspace_at(1.0, 0.0, 200, m - 1);
+    const double z1 = sqrt(1.0 - mu1 * mu1), z0 = sqrt(1.0 - mu0 * mu0);
+    return (z1 - z0) * (z1 * ld_law(law, mu1, c, nc) + z0 * ld_law(law, mu0, c, nc)) / 2.0;
+}
+
+// Sum over a wavefront (64 lanes) by butterfly shuffles; every lane gets the total.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
 }  // namespace rr

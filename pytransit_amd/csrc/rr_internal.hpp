@@ -9,17 +9,19 @@
 
 namespace rr {
 
-// Per-parameter-vector state written by the setup kernels and read by the flux
+// Per-parameter-vector state written by the setup kernel and read by the flux
 // kernels: Taylor coefficients (x then y, divided by n!), the contact window
 // (first/fourth contact, before the per-light-curve margin), period, 1/period,
-// the radius ratio the window and the k-table lerp used, and a validity flag.
+// the radius ratio the window and the k-table lerp used, and two validity flags
+// (orbit parameters valid; limb-darkening profile not NaN), written by the two
+// block roles of the setup kernel.
 struct alignas(16) PvRec {
     double c[10];
     double bt1, bt4;
     double p, inv_p;
     double k0;
     int32_t good;
-    int32_t pad;
+    int32_t ldok;
 };
 static_assert(sizeof(PvRec) == 128, "PvRec must be 128 bytes");
 

@@ -3,25 +3,34 @@
 //
 // Data layout in HBM (all row-major):
 //   times[npt] f64, lcids[npt] i32, per-lc arrays [nlc]
-//   k[npv, kcols], t0[npv, nep], p/a/inc/e/w[npv] f64
-//   ldp[npv, npb, nz] f64, istar[npv, npb] f64
-//   workspace: PvRec[npv] (128 B each) | ldm64[npv, npb, ng] | ldm32[npv, npb, ng]
+//   k[npv, kcols], t0[npv, nep], p/a/inc/e/w[npv] f64, ldc[npv, npb, nldc] f64
+//   weight table W[nk, nz, ng] f64 (+ f32 copy), g fastest
+//   workspace: PvRec[npv] (128 B) | istar[npv, npb] f64 | ldm64[npv, npb, ng] | ldm32[npv, npb, ng]
 //   flux[npv, npt] f64 (RR_FP64) or f32 (RR_MIXED)
 //
-// The flux kernel maps one workgroup of 256 lanes to one parameter vector and a
-// tile of 1024 consecutive times, so a wavefront reads 64 consecutive times
-// (coalesced) and writes 64 consecutive flux values; consecutive times share the
-// in/out-of-window branch, so wavefronts rarely diverge.
+// Setup is one launch with two block roles: orbit blocks (8 pv per 64-lane
+// block, the 7 Kepler solves of a pv on 7 lanes) and limb-darkening blocks (one
+// pv per block: profiles at the mu nodes in LDS, disk integral, and the
+// table-lerp contraction with lanes over g reading the table coalesced).
+//
+// The flux kernel runs one wavefront per workgroup over a tile of 1024 times of
+// one pv; each lane owns two runs of 8 consecutive times and issues all its time
+// loads up front.  With sorted times and one light curve, a lane classifies each
+// run from the phases of its first and last time: runs entirely outside the
+// transit window store 1.0 as vectors.  In-window points are compacted into a
+// per-wave LDS queue (ballot + popcount, count kept in a register), and all 64
+// lanes then share the supersampled geometry of the queued points, reading the
+// limb-darkening row from LDS.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "rr_device.hpp"
 #include "rr_internal.hpp"
-#include "../../include/rr_hip.h"
 
 namespace rr {
 
@@ -46,8 +55,8 @@ int check_launch(const char* what) {
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // ------------------------------------------------------------------------------------
-// Weight table; common.py:188-223.  One lane per (k, g) row, areas accumulated
-// left to right as in the reference so the normalising sum rounds the same way.
+// Weight table; common.py:188-223.  One lane per (k, g) row, the areas of a row
+// accumulated left to right as in the reference; written as W[ik][iz][ig].
 // ------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) weights_3d_kernel(int nk, double kmin, double kmax, const double* __restrict__ ze,
                                                          int nz, int ng, double* __restrict__ W,
@@ -58,99 +67,46 @@ __global__ void __launch_bounds__(256) weights_3d_kernel(int nk, double kmin, do
     const double k = linspace_at(kmin, kmax, nk, ik);
     const double g = linspace_at(0.0, 1.0 - 1e-7, ng, ig);
     const double b = g * (1.0 + k);
-    double* w = W + (size_t)row * nz;
+    const size_t base = (size_t)ik * nz * ng + ig;
     double a0 = cci_area(ze[0], k, b);
-    w[0] = a0;
     double s = a0;
+    W[base] = a0;
     for (int i = 1; i < nz; ++i) {
         const double a1 = cci_area(ze[i], k, b);
-        w[i] = a1 - a0;
+        const double wi = a1 - a0;
+        W[base + (size_t)i * ng] = wi;
         a0 = a1;
-        s += w[i];
+        s += wi;
     }
     for (int i = 0; i < nz; ++i) {
-        const double v = w[i] / s;
-        w[i] = v;
-        if (W32) W32[(size_t)row * nz + i] = (float)v;
+        const double v = W[base + (size_t)i * ng] / s;
+        W[base + (size_t)i * ng] = v;
+        if (W32) W32[base + (size_t)i * ng] = (float)v;
     }
 }
 
 // ------------------------------------------------------------------------------------
-// Limb-darkening laws; models/numba/ldmodels.py:22-139.
+// Limb-darkening profiles for the rr_ld_profiles entry point (one lane per value;
+// lane iz == 0 also computes the disk integral).
 // ------------------------------------------------------------------------------------
-__device__ double ld_law(int law, double mu, const double* c, int nc) {
-    switch (law) {
-        case RR_LD_UNIFORM: return 1.0;
-        case RR_LD_LINEAR: return 1.0 - c[0] * (1.0 - mu);
-        case RR_LD_QUADRATIC: return 1.0 - c[0] * (1.0 - mu) - c[1] * ((1.0 - mu) * (1.0 - mu));
-        case RR_LD_QUADRATIC_TRI: {
-            const double a = sqrt(c[0]), b = 2.0 * c[1];
-            const double u = a * b, v = a * (1.0 - b);
-            return 1.0 - u * (1.0 - mu) - v * ((1.0 - mu) * (1.0 - mu));
-        }
-        case RR_LD_NONLINEAR:
-            return 1.0 - c[0] * (1.0 - sqrt(mu)) - c[1] * (1.0 - mu) - c[2] * (1.0 - pow(mu, 1.5)) -
-                   c[3] * (1.0 - mu * mu);
-        case RR_LD_GENERAL: {
-            double s = 0.0;
-            for (int i = 0; i < nc; ++i) s += c[i] * (1.0 - pow(mu, (double)(i + 1)));
-            return s;
-        }
-        case RR_LD_SQUARE_ROOT: return 1.0 - c[0] * (1.0 - mu) - c[1] * (1.0 - sqrt(mu));
-        case RR_LD_LOGARITHMIC: return 1.0 - c[0] * (1.0 - mu) - c[1] * mu * log(mu);
-        case RR_LD_EXPONENTIAL: return 1.0 - c[0] * (1.0 - mu) - c[1] / (1.0 - exp(mu));
-        case RR_LD_POWER_2: return 1.0 - c[0] * (1.0 - pow(mu, c[1]));
-        case RR_LD_POWER_2_PM: {
-            const double cc = 1.0 - c[0] + c[1];
-            const double a = log2(cc / c[1]);
-            return 1.0 - cc * (1.0 - pow(mu, a));
-        }
-        default: return NAN;
-    }
-}
-
-// Analytic disk integral where the reference has one (ldmodels.py:27-123); returns
-// false for the laws that fall back to the trapezoid (rrmodel.py:220-227, Q8).
-__device__ bool ld_integral(int law, const double* c, double& out) {
-    switch (law) {
-        case RR_LD_UNIFORM: out = kPi; return true;
-        case RR_LD_LINEAR: out = 2.0 * kPi * 1.0 / 6.0 * (3.0 - 2.0 * c[0]); return true;  // Q7, as the reference
-        case RR_LD_QUADRATIC: out = 2.0 * kPi * 1.0 / 12.0 * (-2.0 * c[0] - c[1] + 6.0); return true;
-        case RR_LD_QUADRATIC_TRI: {
-            const double a = sqrt(c[0]), b = 2.0 * c[1];
-            const double u = a * b, v = a * (1.0 - b);
-            out = 2.0 * kPi * 1.0 / 12.0 * (-2.0 * u - v + 6.0);
-            return true;
-        }
-        case RR_LD_POWER_2: out = 2.0 * kPi * (-c[0] * c[1] + c[1] + 2.0) / (2.0 * c[1] + 4.0); return true;
-        default: return false;
-    }
-}
-
 __global__ void __launch_bounds__(256) ld_profiles_kernel(int law, const double* __restrict__ ldc, int npv, int npb,
                                                           int nldc, const double* __restrict__ mu, int nmu,
                                                           double* __restrict__ ldp, double* __restrict__ istar) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= npv * npb) return;
-    const double* c = ldc + (size_t)j * nldc;
-    double* out = ldp + (size_t)j * nmu;
-    for (int i = 0; i < nmu; ++i) out[i] = ld_law(law, mu[i], c, nldc);
-    double is;
-    if (!ld_integral(law, c, is)) {
-        // 2*pi*trapezoid(z*I(mu), z) on mu = linspace(1, 0, 200), z = sqrt(1-mu^2)
-        const int n = 200;
-        double zp = 0.0, yp = 0.0, s = 0.0;
-        for (int m = 0; m < n; ++m) {
-            const double muv = linspace_at(1.0, 0.0, n, m);
-            const double z = sqrt(1.0 - muv * muv);
-            const double y = z * ld_law(law, muv, c, nldc);
-            if (m > 0) s += (z - zp) * (y + yp) / 2.0;
-            zp = z;
-            yp = y;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)npv * npb * nmu) return;
+    const long long j = idx / nmu;
+    const int iz = (int)(idx - j * nmu);
+    const double* c = ldc + j * nldc;
+    ldp[idx] = ld_law(law, mu[iz], c, nldc);
+    if (iz == 0) {
+        double is;
+        if (!ld_integral(law, c, is)) {
+            double s = 0.0;
+            for (int m = 1; m < 200; ++m) s += ld_trapz_segment(law, c, nldc, m);
+            is = 2.0 * kPi * s;
         }
-        is = 2.0 * kPi * s;
+        istar[j] = is;
     }
-    istar[j] = is;
 }
 
 // istar = 2*pi*sum dz*(z_i I_i + z_{i-1} I_{i-1})/2; ldtkldm.py:77-91.
@@ -171,124 +127,184 @@ __global__ void __launch_bounds__(256) integrate_profiles_kernel(const double* _
 }
 
 // ------------------------------------------------------------------------------------
-// Orbit: Taylor coefficients (7 Kepler solves spread over lanes 0-6) and the two
-// contact points (lanes 0-1).  Used by the RR and TS setup kernels.
+// Orbit for 8 parameter vectors per 64-lane block: lane j (0..6) of a group of 8
+// solves Kepler at t = (j-3)*dt, lane 0 forms the stencils, lanes 0 and 1 find
+// the first and fourth contacts.  Writes coefficients[10] and the window.
 // ------------------------------------------------------------------------------------
-__device__ void orbit_setup_block(double p, double a, double inc, double e, double w, double kw, double* s_x,
-                                  double* s_y, double* s_c, double* s_win) {
-    const int lane = threadIdx.x;
-    if (lane < 7) {
+constexpr int kOrbitGroup = 8;
+
+__device__ __forceinline__ void orbit_group(int pv, bool valid, double p, double a, double inc, double e, double w,
+                                            double kw, double* out_c, double* out_win) {
+    __shared__ double s_x[8][8], s_y[8][8], s_c[8][10], s_w[8][2];
+    const int sub = threadIdx.x / kOrbitGroup, j = threadIdx.x % kOrbitGroup;
+    if (valid && j < 7) {
         double x, y;
-        kepler_xy((lane - 3) * 2e-2, p, a, cos(inc), e, w, x, y);
-        s_x[lane] = x;
-        s_y[lane] = y;
+        kepler_xy((j - 3) * 2e-2, p, a, cos(inc), e, w, x, y);
+        s_x[sub][j] = x;
+        s_y[sub][j] = y;
     }
     __syncthreads();
-    if (lane == 0) taylor_from_stencil(s_x, s_y, s_c);
+    if (valid && j == 0) taylor_from_stencil(s_x[sub], s_y[sub], s_c[sub]);
     __syncthreads();
-    if (lane < 2) s_win[lane] = find_contact_point(kw, lane == 0 ? 1 : 4, s_c);
+    if (valid && j < 2) s_w[sub][j] = find_contact_point(kw, j == 0 ? 1 : 4, s_c[sub]);
     __syncthreads();
+    if (valid) {
+        out_c[j] = s_c[sub][j];
+        if (j < 2) out_c[8 + j] = s_c[sub][8 + j];
+        if (j < 2) out_win[j] = s_w[sub][j];
+    }
 }
 
 __global__ void __launch_bounds__(64) orbit_kernel(int npv, const double* __restrict__ P, const double* __restrict__ A,
                                                    const double* __restrict__ I, const double* __restrict__ E,
                                                    const double* __restrict__ Wp, const double* __restrict__ K0,
                                                    double* __restrict__ coeffs, double* __restrict__ win) {
-    __shared__ double s_x[7], s_y[7], s_c[10], s_win[2];
-    const int pv = blockIdx.x;
-    orbit_setup_block(P[pv], A[pv], I[pv], E[pv], Wp[pv], K0[pv], s_x, s_y, s_c, s_win);
-    if (threadIdx.x < 10) coeffs[pv * 10 + threadIdx.x] = s_c[threadIdx.x];
-    if (threadIdx.x < 2) win[pv * 2 + threadIdx.x] = s_win[threadIdx.x];
+    const int pv = blockIdx.x * kOrbitGroup + threadIdx.x / kOrbitGroup;
+    const bool valid = pv < npv;
+    const int q = valid ? pv : 0;
+    orbit_group(pv, valid, P[q], A[q], I[q], E[q], Wp[q], K0[q], coeffs + (size_t)q * 10, win + (size_t)q * 2);
 }
 
 // ------------------------------------------------------------------------------------
-// Per-pv setup of rr_full; model_full.py:39-70.  One 64-lane workgroup per pv.
+// Per-pv setup of rr_full; model_full.py:39-70 and rrmodel.py:215-227.
 // ------------------------------------------------------------------------------------
+constexpr int kMaxNz = 1024;
+
 struct SetupArgs {
     const double* k; const double* p; const double* a; const double* inc; const double* e; const double* w;
-    const double* ldp; const double* W; const double* ze;
-    PvRec* rec; double* ldm64; float* ldm32;
-    int npb, kcols, nk, ng, nz;
+    int law, nldc;
+    const double* ldc; const double* ldp_in; const double* istar_in;
+    const double* W; const float* W32; const double* ze; const double* mu;
+    PvRec* rec; double* istar; double* ldm64; float* ldm32;
+    int npv, npb, kcols, nk, ng, nz, n_orbit_blocks;
     double kmin, kmax, dk;
 };
 
-__global__ void __launch_bounds__(64) rr_setup_kernel(SetupArgs s) {
-    __shared__ double s_x[7], s_y[7], s_c[10], s_win[2];
-    const int pv = blockIdx.x;
-    const int lane = threadIdx.x;
-    const double a = s.a[pv], e = s.e[pv];
-    const double* ldp_pv = s.ldp + (size_t)pv * s.npb * s.nz;
-    const bool good = !(isnan(a) || a <= 1.0 || e < 0.0 || isnan(ldp_pv[0]));
-    PvRec* rec = s.rec + pv;
-    if (!good) {
-        if (lane == 0) { rec->good = 0; }
-        return;
+__device__ void setup_orbit_role(const SetupArgs& s, int blk) {
+    const int pv = blk * kOrbitGroup + threadIdx.x / kOrbitGroup;
+    const int j = threadIdx.x % kOrbitGroup;
+    const bool inrange = pv < s.npv;
+    const int q = inrange ? pv : 0;
+    const double a = s.a[q], e = s.e[q];
+    const bool valid = inrange && !(isnan(a) || a <= 1.0 || e < 0.0);
+    const double k0 = s.k[(size_t)q * s.kcols];
+    PvRec* rec = s.rec + q;
+    orbit_group(pv, valid, s.p[q], a, s.inc[q], e, s.w[q], k0, rec->c, &rec->bt1);
+    if (inrange && j == 0) {
+        rec->p = s.p[q];
+        rec->inv_p = 1.0 / s.p[q];
+        rec->k0 = k0;
+        rec->good = valid ? 1 : 0;
     }
-    const double k0 = s.k[(size_t)pv * s.kcols];
+}
 
-    // Limb-darkening means: lerp in k of two table rows, or exact 2-D weights
-    // when k0 is outside [kmin, kmax]; model_full.py:47-55.
-    const size_t ldm_base = (size_t)pv * s.npb * s.ng;
-    if (s.kmin <= k0 && k0 <= s.kmax) {
-        int ik = (int)floor((k0 - s.kmin) / s.dk);
+__device__ void setup_ld_role(const SetupArgs& s, int pv) {
+    __shared__ double s_l[kMaxNz];
+    const int lane = threadIdx.x;
+    const int nz = s.nz, ng = s.ng;
+    const double k0 = s.k[(size_t)pv * s.kcols];
+    const bool in_table = s.W != nullptr && s.kmin <= k0 && k0 <= s.kmax;
+    int ik = 0;
+    double ak = 0.0;
+    if (in_table) {
+        ik = (int)floor((k0 - s.kmin) / s.dk);
         ik = ik > s.nk - 2 ? s.nk - 2 : ik;
-        const double ak = (k0 - s.kmin - ik * s.dk) / s.dk;
-        const double* W0 = s.W + (size_t)ik * s.ng * s.nz;
-        const double* W1 = W0 + (size_t)s.ng * s.nz;
-        for (int ipb = 0; ipb < s.npb; ++ipb) {
-            const double* l = ldp_pv + (size_t)ipb * s.nz;
-            for (int ig = lane; ig < s.ng; ig += 64) {
-                double d0 = 0.0, d1 = 0.0;
-                for (int iz = 0; iz < s.nz; ++iz) {
-                    d0 += W0[ig * s.nz + iz] * l[iz];
-                    d1 += W1[ig * s.nz + iz] * l[iz];
-                }
-                const double v = (1.0 - ak) * d0 + ak * d1;
-                s.ldm64[ldm_base + (size_t)ipb * s.ng + ig] = v;
-                if (s.ldm32) s.ldm32[ldm_base + (size_t)ipb * s.ng + ig] = (float)v;
+        ak = (k0 - s.kmin - ik * s.dk) / s.dk;
+    }
+    const bool mixed = s.ldm32 != nullptr;
+    for (int ipb = 0; ipb < s.npb; ++ipb) {
+        const size_t j = (size_t)pv * s.npb + ipb;
+        // profile at the mu nodes and its disk integral
+        double ist;
+        if (s.law == RR_LD_TABULATED) {
+            for (int iz = lane; iz < nz; iz += 64) s_l[iz] = s.ldp_in[j * nz + iz];
+            ist = s.istar_in[j];
+        } else {
+            const double* c = s.ldc + j * s.nldc;
+            for (int iz = lane; iz < nz; iz += 64) s_l[iz] = ld_law(s.law, s.mu[iz], c, s.nldc);
+            if (!ld_integral(s.law, c, ist)) {
+                double part = 0.0;
+                for (int m = 1 + lane; m < 200; m += 64) part += ld_trapz_segment(s.law, c, s.nldc, m);
+                ist = 2.0 * kPi * wave_sum(part);
             }
         }
-    } else {
-        for (int ig = lane; ig < s.ng; ig += 64) {
-            const double g = linspace_at(0.0, 1.0 - 1e-7, s.ng, ig);
-            const double b = g * (1.0 + k0);
-            // pass 1: normalising sum of the annulus weights (common.py:152-185)
-            double a0 = cci_area(s.ze[0], k0, b), sum = a0;
-            for (int iz = 1; iz < s.nz; ++iz) {
-                const double a1 = cci_area(s.ze[iz], k0, b);
-                sum += a1 - a0;
-                a0 = a1;
-            }
-            for (int ipb = 0; ipb < s.npb; ++ipb) {
-                const double* l = ldp_pv + (size_t)ipb * s.nz;
+        __syncthreads();
+        if (lane == 0) {
+            s.istar[j] = ist;
+            if (ipb == 0) s.rec[pv].ldok = isnan(s_l[0]) ? 0 : 1;
+        }
+        // limb-darkening means: lerp in k of two table rows, or exact 2-D weights
+        // when k0 is outside [kmin, kmax]; model_full.py:47-55
+        for (int ig = lane; ig < ng; ig += 64) {
+            double v;
+            if (in_table) {
+                double d0 = 0.0, d1 = 0.0;
+                if (mixed && s.W32) {
+                    const float* w0 = s.W32 + (size_t)ik * nz * ng + ig;
+                    const float* w1 = w0 + (size_t)nz * ng;
+                    for (int iz = 0; iz < nz; ++iz) {
+                        d0 += (double)w0[(size_t)iz * ng] * s_l[iz];
+                        d1 += (double)w1[(size_t)iz * ng] * s_l[iz];
+                    }
+                } else {
+                    const double* w0 = s.W + (size_t)ik * nz * ng + ig;
+                    const double* w1 = w0 + (size_t)nz * ng;
+                    for (int iz = 0; iz < nz; ++iz) {
+                        d0 += w0[(size_t)iz * ng] * s_l[iz];
+                        d1 += w1[(size_t)iz * ng] * s_l[iz];
+                    }
+                }
+                v = (1.0 - ak) * d0 + ak * d1;
+            } else {
+                // common.py:152-185: normalised annulus areas, recomputed in a second pass
+                const double g = linspace_at(0.0, 1.0 - 1e-7, ng, ig);
+                const double b = g * (1.0 + k0);
+                double a0 = cci_area(s.ze[0], k0, b), sum = a0;
+                for (int iz = 1; iz < nz; ++iz) {
+                    const double a1 = cci_area(s.ze[iz], k0, b);
+                    sum += a1 - a0;
+                    a0 = a1;
+                }
                 double b0 = cci_area(s.ze[0], k0, b);
-                double d = (b0 / sum) * l[0];
-                for (int iz = 1; iz < s.nz; ++iz) {
+                v = (b0 / sum) * s_l[0];
+                for (int iz = 1; iz < nz; ++iz) {
                     const double b1 = cci_area(s.ze[iz], k0, b);
-                    d += ((b1 - b0) / sum) * l[iz];
+                    v += ((b1 - b0) / sum) * s_l[iz];
                     b0 = b1;
                 }
-                s.ldm64[ldm_base + (size_t)ipb * s.ng + ig] = d;
-                if (s.ldm32) s.ldm32[ldm_base + (size_t)ipb * s.ng + ig] = (float)d;
             }
+            if (mixed) s.ldm32[j * ng + ig] = (float)v;
+            else s.ldm64[j * ng + ig] = v;
         }
+        __syncthreads();
     }
+}
 
-    // Taylor orbit and contact window; model_full.py:60-67.
-    orbit_setup_block(s.p[pv], a, s.inc[pv], e, s.w[pv], k0, s_x, s_y, s_c, s_win);
-    if (lane < 10) rec->c[lane] = s_c[lane];
-    if (lane == 0) {
-        rec->bt1 = s_win[0];
-        rec->bt4 = s_win[1];
-        rec->p = s.p[pv];
-        rec->inv_p = 1.0 / s.p[pv];
-        rec->k0 = k0;
-        rec->good = 1;
-    }
+__global__ void __launch_bounds__(64) rr_setup_kernel(SetupArgs s) {
+    if ((int)blockIdx.x < s.n_orbit_blocks) setup_orbit_role(s, blockIdx.x);
+    else setup_ld_role(s, blockIdx.x - s.n_orbit_blocks);
 }
 
 // ------------------------------------------------------------------------------------
 // Flux; model_full.py:75-99.
+//
+// One wavefront per workgroup covers a tile of 1024 times of one pv.  Lane l owns
+// two runs of 8 consecutive times (run step m of the wave covers 512 contiguous
+// times) and issues its time loads up front.
+//
+// Phase A: the FP64 epoch reduction classifies points against the transit window.
+// With sorted times and one light curve a lane classifies a whole run from the
+// phases of its first and last time; when the whole wave's run step is outside
+// the window (the common case) the wave stores 1.0 cooperatively, each store
+// instruction covering 1 KB of contiguous memory.  In-window points go to an LDS
+// queue (ballot + popcount; the count is wave-uniform and kept in a register).
+//
+// Phase B: with one nsamples for all light curves, work items are (point, sample)
+// pairs in sample order, so a round of 64 lanes covers ~64/nss neighbouring
+// points and the branch mix (outside / inside / limb) stays coherent; per-point
+// sums are formed in sample order from LDS, so results do not depend on the
+// schedule.  Otherwise one lane takes one point.  Waves in phase A (HBM-bound)
+// and waves in phase B (VALU-bound) share each CU.
 // ------------------------------------------------------------------------------------
 struct FluxArgs {
     const double* times; const int32_t* lcids; const int32_t* pbids; const int32_t* epids;
@@ -296,97 +312,280 @@ struct FluxArgs {
     const double* k; const double* t0; const double* istar;
     const PvRec* rec; const double* ldm64; const float* ldm32;
     void* flux;
-    int npt, nlc, npb, nep, kcols, t0cols, ng, ntiles;
+    int npt, nlc, npb, kcols, t0cols, ng, ntiles;
+    int uniform_ns;                   // the common nsamples of all light curves, or 0
+    int chunk_pts;                    // points per phase-B chunk (uniform_ns > 0)
+    int row_lds;                      // stage the limb-darkening row in LDS (one light curve)
     double dg;
 };
 
-constexpr int kFluxThreads = 256;
-constexpr int kFluxItems = 4;
-constexpr int kFluxTile = kFluxThreads * kFluxItems;
+constexpr int kThreads = 64;
+constexpr int kRun = 8;
+constexpr int kRuns = 2;
+constexpr int kTile = kThreads * kRun * kRuns;       // 1024 times per workgroup
+constexpr int kMaxItems = 1024;                       // (point, sample) items per LDS chunk
+constexpr int kMaxChunk = 64;
 
-template <bool kMixed, bool kOneLC>
-__global__ void __launch_bounds__(kFluxThreads) rr_flux_kernel(FluxArgs f) {
+// Bytes of dynamic LDS the flux kernel uses for these parameters.
+static inline size_t flux_lds_bytes(bool mixed, int ng, int chunk_pts, int uniform_ns, bool row_lds) {
+    const size_t real = mixed ? 4 : 8;
+    size_t b = (size_t)kTile * sizeof(uint16_t);                         // queue
+    b += (size_t)kMaxChunk * sizeof(double);                             // phases of a chunk
+    if (uniform_ns > 0) b += (size_t)chunk_pts * uniform_ns * real;      // item results
+    if (row_lds) b += (size_t)ng * real;                                 // limb-darkening row
+    return (b + 15) / 16 * 16;
+}
+
+// Epoch and phase of time t (the division-free form of model_full.py:88-89).
+__device__ __forceinline__ double phase_of(double t, double t0, double p, double inv_p, double& epoch) {
+#pragma clang fp contract(off)
+    const double u = t - t0 + 0.5 * p;
+    double ep = floor(u * inv_p);
+    const double r = u - ep * p;
+    ep += (r < 0.0) ? -1.0 : ((r >= p) ? 1.0 : 0.0);
+    epoch = ep;
+    return t - (t0 + ep * p);
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// out[i0 .. i0+n) = v by the 64 lanes of a wave: 16-byte non-temporal vectors,
+// consecutive lanes on consecutive vectors, so each store instruction covers 1 KB
+// of contiguous memory.
+template <typename T>
+__device__ __forceinline__ void wave_fill(T* out, int i0, int n, T v, int wl) {
+    constexpr int kVec = 16 / sizeof(T);
+    if (n <= 0) return;
+    T* base = out + i0;
+    int head = (int)(((16 - ((uintptr_t)base & 15)) & 15) / sizeof(T));
+    if (((uintptr_t)base % sizeof(T)) != 0 || head > n) head = n;
+    for (int i = wl; i < head; i += 64) base[i] = v;
+    T* vb = base + head;
+    const int nv = (n - head) / kVec;
+    if constexpr (std::is_same<T, float>::value) {
+        const f32x4 vv = {v, v, v, v};
+        for (int i = wl; i < nv; i += 64) __builtin_nontemporal_store(vv, (f32x4*)vb + i);
+    } else {
+        const f64x2 vv = {v, v};
+        for (int i = wl; i < nv; i += 64) __builtin_nontemporal_store(vv, (f64x2*)vb + i);
+    }
+    for (int i = head + nv * kVec + wl; i < n; i += 64) base[i] = v;
+}
+
+// Per-light-curve quantities of one point (model_full.py:84-86, 93-98).
+struct PointCtx {
+    int ipb, ns;
+    double ext, kk, ist, t0v, lo, hi;
+};
+
+__device__ __forceinline__ PointCtx point_ctx(const FluxArgs& f, const PvRec& r, int pv, int ilc) {
+    PointCtx c;
+    c.ipb = min(max(f.pbids[ilc], 0), f.npb - 1);
+    c.ns = f.nsamples[ilc];
+    c.ext = f.exptimes[ilc];
+    c.kk = f.k[(size_t)pv * f.kcols + (f.kcols == 1 ? 0 : c.ipb)];
+    c.ist = f.istar[(size_t)pv * f.npb + c.ipb];
+    c.t0v = f.t0[(size_t)pv * f.t0cols + min(max(f.epids[ilc], 0), f.t0cols - 1)];
+    const double margin = 0.003 + c.ext;
+    c.lo = r.bt1 - margin;
+    c.hi = r.bt4 + margin;
+    return c;
+}
+
+// Contribution of sample `is` (0-based) of a point at phase tc: FP64 gives the
+// reference's (istar - I*A)/istar, FP32 gives the depth I*A (the flux is
+// 1 - mean(depth)/istar).
+__device__ __forceinline__ double sample_f64(const double* cc, double dg, int ng, const PointCtx& c,
+                                             const double* lrow, double tc, int is) {
+    const double off = c.ext * ((is + 1 - 0.5) / c.ns - 0.5);
+    const double z = taylor_z<double>(tc + off, cc);
+    if (!(z < 1.0 + c.kk)) return 1.0;  // ccia gives 0 and interp 0 for z >= 1+k: (ist - 0)/ist
+    const double ip = interp_ld(z / (1.0 + c.kk), dg, lrow, ng);
+    double ap, kap;
+    ccia_kite<double>(1.0, c.kk, z, ap, kap);
+    return (c.ist - ip * ap) / c.ist;
+}
+
+__device__ __forceinline__ float sample_f32(const float* cf, float kf, float inv_dg, float tcf, float extf,
+                                            float inv_ns, const float* lrow, int ng, int is) {
+    const float zlim = 1.0f + kf;
+    const float off = extf * (((float)is + 0.5f) * inv_ns - 0.5f);
+    const float z = taylor_z<float>(tcf + off, cf);
+    if (!(z < zlim)) return 0.0f;
+    const float ip = interp_ld_f(z * (1.0f / zlim), inv_dg, lrow, ng);
+    float ap, kap;
+    ccia_kite<float>(1.0f, kf, z, ap, kap);
+    return ip * ap;
+}
+
+template <bool kMixed, bool kFast>
+__global__ void __launch_bounds__(kThreads) rr_flux_kernel(FluxArgs f) {
+    using Real = typename std::conditional<kMixed, float, double>::type;
+    extern __shared__ __align__(16) unsigned char lds[];
+    uint16_t* s_q = (uint16_t*)lds;
+    double* s_tc = (double*)(lds + kTile * sizeof(uint16_t));
+    Real* s_dep = (Real*)(s_tc + kMaxChunk);
+    Real* s_row = s_dep + (f.uniform_ns > 0 ? f.chunk_pts * f.uniform_ns : 0);
+
     const int pv = blockIdx.x / f.ntiles;
     const int tile = blockIdx.x - pv * f.ntiles;
-    const int base = tile * kFluxTile + threadIdx.x;
+    const int wl = threadIdx.x;
+    const int tbase = tile * kTile;
+    int its[kRuns], nrun[kRuns];
+#pragma unroll
+    for (int m = 0; m < kRuns; ++m) {
+        its[m] = tbase + (m * kThreads + wl) * kRun;
+        nrun[m] = min(kRun, f.npt - its[m]);  // <= 0 past the end
+    }
+    // issue the time loads before anything depends on the per-pv record
+    double tf[kRuns], tl[kRuns];
+#pragma unroll
+    for (int m = 0; m < kRuns; ++m) {
+        tf[m] = nrun[m] > 0 ? f.times[its[m]] : 0.0;
+        tl[m] = nrun[m] > 0 ? f.times[its[m] + nrun[m] - 1] : 0.0;
+    }
     const PvRec& r = f.rec[pv];
-    const size_t out0 = (size_t)pv * f.npt;
+    Real* out = (Real*)f.flux + (size_t)pv * f.npt;
+    if (!(r.good && r.ldok)) {
+        wave_fill<Real>(out, tbase, min(kTile, f.npt - tbase), Real(NAN), wl);
+        return;
+    }
+    const double p = r.p, inv_p = r.inv_p;
+    int count = 0;
+    auto push = [&](bool pred, int it) {
+        const unsigned long long mk = __ballot(pred);
+        if (pred) s_q[count + __popcll(mk & ((1ull << wl) - 1ull))] = (uint16_t)(it - tbase);
+        count += __popcll(mk);
+    };
 
-    if (!r.good) {
-        for (int j = 0; j < kFluxItems; ++j) {
-            const int it = base + j * kFluxThreads;
-            if (it >= f.npt) break;
-            if (kMixed) ((float*)f.flux)[out0 + it] = NAN;
-            else ((double*)f.flux)[out0 + it] = NAN;
+    // ---- phase A -----------------------------------------------------------------
+    if (kFast) {
+        const PointCtx c = point_ctx(f, r, pv, 0);
+#pragma unroll
+        for (int m = 0; m < kRuns; ++m) {
+            bool all_out = true;
+            if (nrun[m] > 0) {
+                double ef, el;
+                const double tcf = phase_of(tf[m], c.t0v, p, inv_p, ef);
+                const double tcl = phase_of(tl[m], c.t0v, p, inv_p, el);
+                all_out = (ef == el) ? (tcl < c.lo || tcf > c.hi)
+                                     : ((el == ef + 1.0) ? (tcf > c.hi && tcl < c.lo) : false);
+            }
+            const int step0 = tbase + m * kThreads * kRun;
+            if (__ballot(!all_out) == 0ull) {  // the wave's whole run step is outside the window
+                wave_fill<Real>(out, step0, min(kThreads * kRun, f.npt - step0), Real(1), wl);
+                continue;
+            }
+            if (all_out && nrun[m] > 0)
+                for (int j = 0; j < nrun[m]; ++j) out[its[m] + j] = Real(1);
+            for (int j = 0; j < kRun; ++j) {
+                bool inw = false;
+                const int it = its[m] + j;
+                if (!all_out && j < nrun[m]) {
+                    double ep;
+                    const double tc = phase_of(f.times[it], c.t0v, p, inv_p, ep);
+                    inw = (c.lo <= tc) && (tc <= c.hi);
+                    if (!inw) out[it] = Real(1);
+                }
+                push(inw, it);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < kRuns; ++m) {
+            for (int j = 0; j < kRun; ++j) {
+                bool inw = false;
+                const int it = its[m] + j;
+                if (j < nrun[m]) {
+                    const int ilc = f.nlc == 1 ? 0 : min(max(f.lcids[it], 0), f.nlc - 1);
+                    const PointCtx c = point_ctx(f, r, pv, ilc);
+                    double ep;
+                    const double tc = phase_of(f.times[it], c.t0v, p, inv_p, ep);
+                    inw = (c.lo <= tc) && (tc <= c.hi);
+                    if (!inw) out[it] = Real(1);
+                }
+                push(inw, it);
+            }
+        }
+    }
+    const int n = count;
+    if (n == 0) return;
+
+    // ---- phase B -----------------------------------------------------------------
+    if (f.row_lds) {
+        const int ipb = min(max(f.pbids[0], 0), f.npb - 1);
+        const size_t row = ((size_t)pv * f.npb + ipb) * f.ng;
+        for (int i = wl; i < f.ng; i += kThreads) s_row[i] = kMixed ? (Real)f.ldm32[row + i] : (Real)f.ldm64[row + i];
+    }
+    __syncthreads();  // queue entries and the staged row are read by other lanes
+    const float inv_dg = (float)(1.0 / f.dg);
+    if (f.uniform_ns > 0 && (kFast || f.nlc == 1)) {
+        // one light curve: the per-point context is uniform
+        const int ns = f.uniform_ns;
+        const PointCtx c = point_ctx(f, r, pv, 0);
+        const size_t row = ((size_t)pv * f.npb + c.ipb) * f.ng;
+        const float inv_ns = 1.0f / (float)ns;
+        const float kf = (float)c.kk, extf = (float)c.ext;
+        float cf[10];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) cf[k] = (float)r.c[k];
+        const int cp = f.chunk_pts;
+        for (int c0 = 0; c0 < n; c0 += cp) {
+            const int cn = min(cp, n - c0);
+            if (wl < cn) {
+                double ep;
+                s_tc[wl] = phase_of(f.times[tbase + s_q[c0 + wl]], c.t0v, p, inv_p, ep);
+            }
+            __syncthreads();
+#pragma unroll 1
+            for (int i = wl; i < cn * ns; i += kThreads) {
+                const int qq = (int)(((float)i + 0.5f) * inv_ns);  // exact i / ns for i, ns <= 1024
+                const int is = i - qq * ns;
+                if (kMixed) {
+                    const float* lrow = f.row_lds ? (const float*)s_row : f.ldm32 + row;
+                    s_dep[i] = sample_f32(cf, kf, inv_dg, (float)s_tc[qq], extf, inv_ns, lrow, f.ng, is);
+                } else {
+                    const double* lrow = f.row_lds ? (const double*)s_row : f.ldm64 + row;
+                    s_dep[i] = sample_f64(r.c, f.dg, f.ng, c, lrow, s_tc[qq], is);
+                }
+            }
+            __syncthreads();
+            if (wl < cn) {
+                Real acc = Real(0);
+                for (int is = 0; is < ns; ++is) acc += s_dep[wl * ns + is];
+                const int it = tbase + s_q[c0 + wl];
+                if (kMixed) out[it] = Real(1.0f - (float)acc * ((float)(1.0 / c.ist) * inv_ns));
+                else out[it] = Real(acc / ns);
+            }
+            __syncthreads();
         }
         return;
     }
-
-    const double p = r.p, inv_p = r.inv_p;
-    float cf[10];
-    if (kMixed) {
-#pragma unroll
-        for (int i = 0; i < 10; ++i) cf[i] = (float)r.c[i];
-    }
-    const float inv_dg = (float)(1.0 / f.dg);
-
-    for (int j = 0; j < kFluxItems; ++j) {
-        const int it = base + j * kFluxThreads;
-        if (it >= f.npt) break;
-        const int ilc = kOneLC ? 0 : min(max(f.lcids[it], 0), f.nlc - 1);
-        const int ipb = min(max(f.pbids[ilc], 0), f.npb - 1);
-        const int iep = min(max(f.epids[ilc], 0), f.t0cols - 1);
-        const double ext = f.exptimes[ilc];
-        const double t0v = f.t0[(size_t)pv * f.t0cols + iep];
-        const double t = f.times[it];
-        const double tc = kMixed ? phase_fast(t, t0v, p, inv_p) : phase_exact(t, t0v, p);
-        const double margin = 0.003 + ext;
-        const bool inwin = (r.bt1 - margin <= tc) && (tc <= r.bt4 + margin);
-        if (!inwin) {
-            if (kMixed) ((float*)f.flux)[out0 + it] = 1.0f;
-            else ((double*)f.flux)[out0 + it] = 1.0;
-            continue;
-        }
-        const int ns = f.nsamples[ilc];
-        const double kk = f.k[(size_t)pv * f.kcols + (f.kcols == 1 ? 0 : ipb)];
-        const double ist = f.istar[(size_t)pv * f.npb + ipb];
-        const size_t row = ((size_t)pv * f.npb + ipb) * f.ng;
+    // general case: one queued point per lane, per-point light-curve context
+#pragma unroll 1
+    for (int q = wl; q < n; q += kThreads) {
+        const int it = tbase + s_q[q];
+        const int ilc = f.nlc == 1 ? 0 : min(max(f.lcids[it], 0), f.nlc - 1);
+        const PointCtx c = point_ctx(f, r, pv, ilc);
+        double ep;
+        const double tc = phase_of(f.times[it], c.t0v, p, inv_p, ep);
+        const size_t row = ((size_t)pv * f.npb + c.ipb) * f.ng;
         if (kMixed) {
-            const float kf = (float)kk;
-            const float zlim = 1.0f + kf;
-            const float inv1pk = 1.0f / zlim;
-            const float tcf = (float)tc, extf = (float)ext;
-            const float inv_ns = 1.0f / (float)ns;
-            const float* lrow = f.ldm32 + row;
+            float cf[10];
+#pragma unroll
+            for (int k = 0; k < 10; ++k) cf[k] = (float)r.c[k];
+            const float inv_ns = 1.0f / (float)c.ns;
+            const float* lrow = f.row_lds ? (const float*)s_row : f.ldm32 + row;
             float depth = 0.0f;
-            for (int is = 0; is < ns; ++is) {
-                const float off = extf * (((float)is + 0.5f) * inv_ns - 0.5f);
-                const float z = taylor_z<float>(tcf + off, cf);
-                if (z < zlim) {
-                    const float ip = interp_ld_f(z * inv1pk, inv_dg, lrow, f.ng);
-                    float ap, kap;
-                    ccia_kite<float>(1.0f, kf, z, ap, kap);
-                    depth = fmaf(ip, ap, depth);
-                }
-            }
-            const float scale = (float)(1.0 / ist) * inv_ns;
-            ((float*)f.flux)[out0 + it] = 1.0f - depth * scale;
+            for (int is = 0; is < c.ns; ++is)
+                depth += sample_f32(cf, (float)c.kk, inv_dg, (float)tc, (float)c.ext, inv_ns, lrow, f.ng, is);
+            out[it] = Real(1.0f - depth * ((float)(1.0 / c.ist) * inv_ns));
         } else {
-            const double* lrow = f.ldm64 + row;
+            const double* lrow = f.row_lds ? (const double*)s_row : f.ldm64 + row;
             double acc = 0.0;
-            for (int is = 1; is <= ns; ++is) {
-                const double off = ext * ((is - 0.5) / ns - 0.5);
-                const double z = taylor_z<double>(tc + off, r.c);
-                if (z < 1.0 + kk) {
-                    const double ip = interp_ld(z / (1.0 + kk), f.dg, lrow, f.ng);
-                    double ap, kap;
-                    ccia_kite<double>(1.0, kk, z, ap, kap);
-                    acc += (ist - ip * ap) / ist;
-                } else {
-                    acc += 1.0;  // ccia gives 0 and interp 0 for z >= 1+k: (ist - 0)/ist
-                }
-            }
-            ((double*)f.flux)[out0 + it] = acc / ns;
+            for (int is = 0; is < c.ns; ++is) acc += sample_f64(r.c, f.dg, f.ng, c, lrow, tc, is);
+            out[it] = Real(acc / c.ns);
         }
     }
 }
@@ -402,7 +601,7 @@ extern "C" {
 
 const char* rr_last_error(void) { return g_last_error.c_str(); }
 
-int rr_abi_version(void) { return 1; }
+int rr_abi_version(void) { return 2; }
 
 int rr_z_grid(double zcut, int nin, int nedge, double* ze, double* zm, double* mu) {
     if (nin < 1 || nedge < 1 || !ze || !zm) return set_error(RR_EINVAL, "rr_z_grid: bad arguments");
@@ -437,18 +636,19 @@ int rr_weights_3d(int nk, double kmin, double kmax, const double* ze, int nz, in
     return check_launch("weights_3d_kernel");
 }
 
+static const int kMinCoeffs[] = {0, 1, 2, 2, 4, 0, 2, 2, 2, 2, 2};
+
 int rr_ld_profiles(int law, const double* ldc, int npv, int npb, int nldc, const double* mu, int nmu, double* ldp,
                    double* istar, void* stream) {
     if (law < RR_LD_UNIFORM || law > RR_LD_POWER_2_PM) return set_error(RR_EINVAL, "rr_ld_profiles: unknown law %d", law);
     if (npv < 0 || npb < 1 || nmu < 1 || nldc < 0) return set_error(RR_EINVAL, "rr_ld_profiles: bad sizes");
-    static const int min_coeffs[] = {0, 1, 2, 2, 4, 0, 2, 2, 2, 2, 2};
-    if (nldc < min_coeffs[law])
-        return set_error(RR_EINVAL, "rr_ld_profiles: law %d needs %d coefficients, got %d", law, min_coeffs[law], nldc);
+    if (nldc < kMinCoeffs[law])
+        return set_error(RR_EINVAL, "rr_ld_profiles: law %d needs %d coefficients, got %d", law, kMinCoeffs[law], nldc);
     if (npv == 0) return RR_OK;
     if (!mu || !ldp || !istar || (nldc > 0 && !ldc)) return set_error(RR_EINVAL, "rr_ld_profiles: null pointer");
-    const int n = npv * npb;
-    hipLaunchKernelGGL(ld_profiles_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, law, ldc, npv,
-                       npb, nldc, mu, nmu, ldp, istar);
+    const long long n = (long long)npv * npb * nmu;
+    hipLaunchKernelGGL(ld_profiles_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, law,
+                       ldc, npv, npb, nldc, mu, nmu, ldp, istar);
     return check_launch("ld_profiles_kernel");
 }
 
@@ -468,18 +668,53 @@ int rr_orbit(int npv, const double* p, const double* a, const double* inc, const
     if (npv < 0) return set_error(RR_EINVAL, "rr_orbit: npv < 0");
     if (npv == 0) return RR_OK;
     if (!p || !a || !inc || !e || !w || !k0 || !coeffs || !win) return set_error(RR_EINVAL, "rr_orbit: null pointer");
-    hipLaunchKernelGGL(orbit_kernel, dim3(npv), dim3(64), 0, (hipStream_t)stream, npv, p, a, inc, e, w, k0, coeffs,
-                       win);
+    hipLaunchKernelGGL(orbit_kernel, dim3((npv + kOrbitGroup - 1) / kOrbitGroup), dim3(64), 0, (hipStream_t)stream,
+                       npv, p, a, inc, e, w, k0, coeffs, win);
     return check_launch("orbit_kernel");
 }
 
-size_t rr_workspace_bytes(int npv, int npb, int ng, int nlc) {
+size_t rr_workspace_bytes(int npv, int npt, int npb, int ng, int nlc) {
     (void)nlc;
-    if (npv < 0 || npb < 0 || ng < 0) return 0;
+    (void)npt;
+    if (npv < 0 || npt < 0 || npb < 0 || ng < 0) return 0;
     size_t b = align_up((size_t)npv * sizeof(PvRec), 256);
+    b += align_up((size_t)npv * npb * sizeof(double), 256);
     b += align_up((size_t)npv * npb * ng * sizeof(double), 256);
     b += align_up((size_t)npv * npb * ng * sizeof(float), 256);
     return b;
+}
+
+struct WsView {
+    PvRec* rec;
+    double* istar;
+    double* ldm64;
+    float* ldm32;
+};
+
+static WsView carve(void* workspace, int npv, int npt, int npb, int ng) {
+    (void)npt;
+    char* ws = (char*)workspace;
+    WsView v;
+    v.rec = (PvRec*)ws;
+    ws += align_up((size_t)npv * sizeof(PvRec), 256);
+    v.istar = (double*)ws;
+    ws += align_up((size_t)npv * npb * sizeof(double), 256);
+    v.ldm64 = (double*)ws;
+    ws += align_up((size_t)npv * npb * ng * sizeof(double), 256);
+    v.ldm32 = (float*)ws;
+    return v;
+}
+
+static int device_cus() {
+    static int cache[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cache[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cache[dev] = n;
+    }
+    return cache[dev];
 }
 
 static int validate(const char* who, int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
@@ -490,93 +725,100 @@ static int validate(const char* who, int precision, const rr_table* tab, const r
     if (npv < 0 || npt < 0 || nlc < 1 || npb < 1 || lc->nep < 1)
         return set_error(RR_EINVAL, "%s: bad sizes (npv=%d npt=%d nlc=%d npb=%d nep=%d)", who, npv, npt, nlc, npb,
                          lc->nep);
+    if (nlc > 32767) return set_error(RR_ERANGE, "%s: at most 32767 light curves", who);
     if (pv->kcols != 1 && pv->kcols != npb)
         return set_error(RR_EINVAL, "Radius ratios should be given either as an [npv, 1] or [npv, npb] array.");
     if (pv->t0cols < 1) return set_error(RR_EINVAL, "%s: t0 needs at least one column", who);
-    if (tab->nk < 2 || tab->ng < 2 || tab->nz < 1 || !tab->weights || !tab->ze)
-        return set_error(RR_EINVAL, "%s: bad weight table", who);
+    if (tab->nk < 2 || tab->ng < 2 || tab->nz < 1 || tab->nz > kMaxNz || !tab->weights || !tab->ze || !tab->mu)
+        return set_error(RR_EINVAL, "%s: bad weight table (nz must be <= %d)", who, kMaxNz);
     if ((long long)npv * npt > (1LL << 40) || (long long)npv * npb * tab->ng > (1LL << 31))
         return set_error(RR_ERANGE, "%s: problem too large", who);
     if (npv == 0 || npt == 0) return RR_OK;
-    const size_t need = rr_workspace_bytes(npv, npb, tab->ng, nlc);
+    const size_t need = rr_workspace_bytes(npv, npt, npb, tab->ng, nlc);
+    if (tab->ng > 8192) return set_error(RR_ERANGE, "%s: ng must be <= 8192", who);
     if (!workspace || workspace_bytes < need)
         return set_error(RR_EINVAL, "%s: workspace of %zu bytes needed, got %zu", who, need, workspace_bytes);
     return RR_OK;
 }
 
-struct WsView {
-    PvRec* rec;
-    double* ldm64;
-    float* ldm32;
-};
-
-static WsView carve(void* workspace, int npv, int npb, int ng) {
-    char* ws = (char*)workspace;
-    WsView v;
-    v.rec = (PvRec*)ws;
-    ws += align_up((size_t)npv * sizeof(PvRec), 256);
-    v.ldm64 = (double*)ws;
-    ws += align_up((size_t)npv * npb * ng * sizeof(double), 256);
-    v.ldm32 = (float*)ws;
-    return v;
-}
-
-int rr_setup(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const double* ldp,
+int rr_setup(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const rr_ld* ld,
              void* workspace, size_t workspace_bytes, void* stream) {
     int rc = validate("rr_setup", precision, tab, lc, pv, workspace_bytes, workspace);
     if (rc) return rc;
     const int npv = pv->npv, npb = lc->npb;
     if (npv == 0 || lc->npt == 0) return RR_OK;
-    if (!pv->k || !pv->p || !pv->a || !pv->inc || !pv->e || !pv->w || !ldp)
+    if (!ld) return set_error(RR_EINVAL, "rr_setup: null limb-darkening descriptor");
+    if (ld->law == RR_LD_TABULATED) {
+        if (!ld->ldp || !ld->istar) return set_error(RR_EINVAL, "rr_setup: tabulated LD needs ldp and istar");
+    } else {
+        if (ld->law < RR_LD_UNIFORM || ld->law > RR_LD_POWER_2_PM)
+            return set_error(RR_EINVAL, "rr_setup: unknown limb-darkening law %d", ld->law);
+        if (ld->nldc < kMinCoeffs[ld->law])
+            return set_error(RR_EINVAL, "rr_setup: law %d needs %d coefficients, got %d", ld->law,
+                             kMinCoeffs[ld->law], ld->nldc);
+        if (ld->nldc > 0 && !ld->ldc) return set_error(RR_EINVAL, "rr_setup: null ldc");
+    }
+    if (!pv->k || !pv->p || !pv->a || !pv->inc || !pv->e || !pv->w)
         return set_error(RR_EINVAL, "rr_setup: null parameter array");
-    WsView v = carve(workspace, npv, npb, tab->ng);
+    WsView v = carve(workspace, npv, lc->npt, npb, tab->ng);
     SetupArgs sa;
     sa.k = pv->k; sa.p = pv->p; sa.a = pv->a; sa.inc = pv->inc; sa.e = pv->e; sa.w = pv->w;
-    sa.ldp = ldp; sa.W = tab->weights; sa.ze = tab->ze;
-    sa.rec = v.rec; sa.ldm64 = v.ldm64; sa.ldm32 = precision == RR_MIXED ? v.ldm32 : nullptr;
-    sa.npb = npb; sa.kcols = pv->kcols; sa.nk = tab->nk; sa.ng = tab->ng; sa.nz = tab->nz;
+    sa.law = ld->law; sa.nldc = ld->nldc; sa.ldc = ld->ldc; sa.ldp_in = ld->ldp; sa.istar_in = ld->istar;
+    sa.W = tab->weights; sa.W32 = tab->weights32; sa.ze = tab->ze; sa.mu = tab->mu;
+    sa.rec = v.rec; sa.istar = v.istar;
+    sa.ldm64 = precision == RR_FP64 ? v.ldm64 : nullptr;
+    sa.ldm32 = precision == RR_MIXED ? v.ldm32 : nullptr;
+    sa.npv = npv; sa.npb = npb; sa.kcols = pv->kcols; sa.nk = tab->nk; sa.ng = tab->ng; sa.nz = tab->nz;
     sa.kmin = tab->kmin; sa.kmax = tab->kmax; sa.dk = tab->dk;
-    hipLaunchKernelGGL(rr_setup_kernel, dim3(npv), dim3(64), 0, (hipStream_t)stream, sa);
+    sa.n_orbit_blocks = (npv + kOrbitGroup - 1) / kOrbitGroup;
+    hipLaunchKernelGGL(rr_setup_kernel, dim3(sa.n_orbit_blocks + npv), dim3(64), 0, (hipStream_t)stream, sa);
     return check_launch("rr_setup_kernel");
 }
 
-int rr_flux(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const double* istar,
-            void* workspace, size_t workspace_bytes, void* flux, void* stream) {
+int rr_flux(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, void* workspace,
+            size_t workspace_bytes, void* flux, void* stream) {
     int rc = validate("rr_flux", precision, tab, lc, pv, workspace_bytes, workspace);
     if (rc) return rc;
     const int npv = pv->npv, npt = lc->npt, npb = lc->npb, nlc = lc->nlc;
     if (npv == 0 || npt == 0) return RR_OK;
     if (!lc->times || !lc->pbids || !lc->epids || !lc->nsamples || !lc->exptimes || (nlc > 1 && !lc->lcids))
         return set_error(RR_EINVAL, "rr_flux: null light-curve array");
-    if (!pv->k || !pv->t0 || !istar || !flux) return set_error(RR_EINVAL, "rr_flux: null parameter array");
-    WsView v = carve(workspace, npv, npb, tab->ng);
+    if (!pv->k || !pv->t0 || !flux) return set_error(RR_EINVAL, "rr_flux: null parameter array");
+    WsView v = carve(workspace, npv, npt, npb, tab->ng);
     FluxArgs fa;
     fa.times = lc->times; fa.lcids = lc->lcids; fa.pbids = lc->pbids; fa.epids = lc->epids;
     fa.nsamples = lc->nsamples; fa.exptimes = lc->exptimes;
-    fa.k = pv->k; fa.t0 = pv->t0; fa.istar = istar;
+    fa.k = pv->k; fa.t0 = pv->t0; fa.istar = v.istar;
     fa.rec = v.rec; fa.ldm64 = v.ldm64; fa.ldm32 = v.ldm32; fa.flux = flux;
-    fa.npt = npt; fa.nlc = nlc; fa.npb = npb; fa.nep = lc->nep; fa.kcols = pv->kcols; fa.t0cols = pv->t0cols;
+    fa.npt = npt; fa.nlc = nlc; fa.npb = npb; fa.kcols = pv->kcols; fa.t0cols = pv->t0cols;
     fa.ng = tab->ng; fa.dg = tab->dg;
-    fa.ntiles = (npt + kFluxTile - 1) / kFluxTile;
+    const bool mixed = precision == RR_MIXED;
+    const int uns = lc->uniform_nsamples;
+    fa.uniform_ns = (uns > 0 && uns <= kMaxItems) ? uns : 0;
+    fa.chunk_pts = fa.uniform_ns > 0 ? (kMaxItems / fa.uniform_ns < kMaxChunk ? kMaxItems / fa.uniform_ns : kMaxChunk) : 0;
+    fa.row_lds = (nlc == 1 && tab->ng <= 2048) ? 1 : 0;
+    fa.ntiles = (npt + kTile - 1) / kTile;
     const long long nblocks = (long long)fa.ntiles * npv;
     if (nblocks > 0x7fffffffLL) return set_error(RR_ERANGE, "rr_flux: grid too large");
+    const size_t lds = flux_lds_bytes(mixed, tab->ng, fa.chunk_pts, fa.uniform_ns, fa.row_lds != 0);
     hipStream_t s = (hipStream_t)stream;
-    const bool one = (nlc == 1);
-    if (precision == RR_MIXED) {
-        if (one) hipLaunchKernelGGL((rr_flux_kernel<true, true>), dim3((unsigned)nblocks), dim3(kFluxThreads), 0, s, fa);
-        else hipLaunchKernelGGL((rr_flux_kernel<true, false>), dim3((unsigned)nblocks), dim3(kFluxThreads), 0, s, fa);
+    const bool fast = (nlc == 1) && lc->sorted;
+    const dim3 grid((unsigned)nblocks), block(kThreads);
+    if (mixed) {
+        if (fast) hipLaunchKernelGGL((rr_flux_kernel<true, true>), grid, block, lds, s, fa);
+        else hipLaunchKernelGGL((rr_flux_kernel<true, false>), grid, block, lds, s, fa);
     } else {
-        if (one) hipLaunchKernelGGL((rr_flux_kernel<false, true>), dim3((unsigned)nblocks), dim3(kFluxThreads), 0, s, fa);
-        else hipLaunchKernelGGL((rr_flux_kernel<false, false>), dim3((unsigned)nblocks), dim3(kFluxThreads), 0, s, fa);
+        if (fast) hipLaunchKernelGGL((rr_flux_kernel<false, true>), grid, block, lds, s, fa);
+        else hipLaunchKernelGGL((rr_flux_kernel<false, false>), grid, block, lds, s, fa);
     }
     return check_launch("rr_flux_kernel");
 }
 
-int rr_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const double* ldp,
-                const double* istar, void* workspace, size_t workspace_bytes, void* flux, void* stream) {
-    int rc = rr_setup(precision, tab, lc, pv, ldp, workspace, workspace_bytes, stream);
+int rr_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const rr_ld* ld,
+                void* workspace, size_t workspace_bytes, void* flux, void* stream) {
+    int rc = rr_setup(precision, tab, lc, pv, ld, workspace, workspace_bytes, stream);
     if (rc) return rc;
-    return rr_flux(precision, tab, lc, pv, istar, workspace, workspace_bytes, flux, stream);
+    return rr_flux(precision, tab, lc, pv, workspace, workspace_bytes, flux, stream);
 }
 
 }  // extern "C"

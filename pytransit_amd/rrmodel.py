@@ -30,7 +30,7 @@ from numpy import isscalar, linspace, ndarray, pi, sqrt, zeros
 from scipy.integrate import trapezoid
 
 from . import _lib
-from ._lib import LD_LAW_IDS, RRLcData, RRParams, RRTable, check, ptr, stream_handle
+from ._lib import LD_LAW_IDS, RRLcData, RRLd, RRParams, RRTable, check, ptr, stream_handle
 from .ldmodel import LDModel
 from .transitmodel import TransitModel
 
@@ -146,31 +146,36 @@ class RoadRunnerModel(TransitModel):
             self._device = torch.device('cuda', torch.cuda.current_device())
         return self._device
 
-    def _table(self, device, need_weights=True):
-        """Device-resident ze and FP64 weight table; (RRTable, keep-alive dict)."""
+    def _table(self, device):
+        """Device-resident z grid, mu nodes and weight table (FP64 and FP32 copies, in
+        the device layout [nk, nz, ng]); returns (RRTable, keep-alive dict)."""
         key = str(device)
         t = self._tables.get(key)
         if t is None:
             lib = _lib.load()
             ze = torch.as_tensor(self.ze, device=device)
+            mu = torch.as_tensor(self.mu, device=device)
             nz = self.ze.size
-            W = torch.empty((self.nk, self.ng, nz), dtype=torch.float64, device=device)
+            W = torch.empty((self.nk, nz, self.ng), dtype=torch.float64, device=device)
+            W32 = torch.empty((self.nk, nz, self.ng), dtype=torch.float32, device=device)
             dk = _lib.c_double()
             dg = _lib.c_double()
             check(lib.rr_weights_3d(self.nk, float(self.klims[0]), float(self.klims[1]), ptr(ze), nz, self.ng,
-                                    ptr(W), 0, _lib.ctypes.byref(dk), _lib.ctypes.byref(dg),
+                                    ptr(W), ptr(W32), _lib.ctypes.byref(dk), _lib.ctypes.byref(dg),
                                     stream_handle(device)))
             self.dk, self.dg = dk.value, dg.value
             tab = RRTable(nk=self.nk, ng=self.ng, nz=nz, pad_=0, kmin=float(self.klims[0]),
-                          kmax=float(self.klims[1]), dk=self.dk, dg=self.dg, weights=ptr(W), ze=ptr(ze))
-            t = (tab, dict(ze=ze, W=W))
+                          kmax=float(self.klims[1]), dk=self.dk, dg=self.dg, weights=ptr(W), weights32=ptr(W32),
+                          ze=ptr(ze), mu=ptr(mu))
+            t = (tab, dict(ze=ze, mu=mu, W=W, W32=W32))
             self._tables[key] = t
         return t
 
     @property
     def weights(self) -> ndarray:
-        """The FP64 weight table W[nk, ng, nz] (a host copy of the device table)."""
-        return self._table(self.device)[1]['W'].cpu().numpy()
+        """The FP64 weight table in the reference's layout W[nk, ng, nz] (a host copy
+        of the device table)."""
+        return self._table(self.device)[1]['W'].permute(0, 2, 1).contiguous().cpu().numpy()
 
     def set_data(self, time: Union[ndarray, List], lcids: Optional[Union[ndarray, List]] = None,
                  pbids: Optional[Union[ndarray, List]] = None, nsamples: Optional[Union[ndarray, List]] = None,
@@ -207,10 +212,24 @@ class RoadRunnerModel(TransitModel):
                     istar[ipv, ipb] = 2 * pi * trapezoid(self._ldz * prof, self._ldz)
         return ldp, istar
 
-    def _ld_profiles(self, ldc, npv: int, npb: int, device):
-        """ldp[npv, npb, nz] and istar[npv, npb] as FP64 device tensors."""
+    def _ld_descriptor(self, ldc, npv: int, npb: int, device):
+        """Limb darkening of one evaluate call as an RRLd descriptor and the device
+        tensors it points to.  Named laws are evaluated by the setup kernel; LDModel
+        plug-ins and host callables give tabulated profiles."""
         lib = _lib.load()
         nz = self.mu.size
+        if self.ldname is not None:
+            law, ncoef = LD_LAW_IDS[self.ldname]
+            shp = _ldc3_shape(_shape_of(ldc))
+            if shp[2] < ncoef:
+                raise ValueError(f"The '{self.ldname}' law needs {ncoef} coefficients per passband, got {shp[2]}")
+            c3 = _as_dev(ldc, device).reshape(shp)
+            if shp[0] not in (1, npv) or shp[1] not in (1, npb):
+                raise ValueError(f"The limb darkening coefficients have shape {shp} after reshaping; expected "
+                                 f"[npv={npv}|1, npb={npb}|1, nldc].")
+            if shp[0] != npv or shp[1] != npb:
+                c3 = c3.expand(npv, npb, shp[2]).contiguous()
+            return RRLd(law=law, nldc=shp[2], ldc=ptr(c3), ldp=0, istar=0), (c3,)
         if isinstance(self.ldmodel, LDModel):
             ldp, istar = self.ldmodel(self.mu, ldc)
             ldp = _as_dev(ldp, device)
@@ -223,27 +242,15 @@ class RoadRunnerModel(TransitModel):
                                                 stream_handle(device)))
             else:
                 istar = _as_dev(istar, device).reshape(ldp.shape[:2])
-        elif self.ldname is not None:
-            law, ncoef = LD_LAW_IDS[self.ldname]
-            shp = _ldc3_shape(_shape_of(ldc))
-            if shp[2] < ncoef:
-                raise ValueError(f"The '{self.ldname}' law needs {ncoef} coefficients per passband, got {shp[2]}")
-            c3 = _as_dev(ldc, device).reshape(shp)
-            key = (str(device), 'mu')
-            mu_d = self._tables.get(key)
-            if mu_d is None:
-                mu_d = torch.as_tensor(self.mu, device=device)
-                self._tables[key] = mu_d
-            ldp = torch.empty((shp[0], shp[1], nz), dtype=torch.float64, device=device)
-            istar = torch.empty((shp[0], shp[1]), dtype=torch.float64, device=device)
-            check(lib.rr_ld_profiles(law, ptr(c3), shp[0], shp[1], shp[2], ptr(mu_d), nz, ptr(ldp), ptr(istar),
-                                     stream_handle(device)))
         else:
             if isinstance(ldc, torch.Tensor):
                 ldc = ldc.detach().cpu().numpy()
             ldp, istar = self._host_ld(ldc, npb)
             ldp, istar = _as_dev(ldp, device), _as_dev(istar, device)
-        return self._broadcast_ld(ldp, istar, npv, npb)
+        if ldp.shape[-1] != nz:
+            raise ValueError(f"The limb darkening profiles have {ldp.shape[-1]} mu nodes, the model has {nz}")
+        ldp, istar = self._broadcast_ld(ldp, istar, npv, npb)
+        return RRLd(law=_lib.RR_LD_TABULATED, nldc=0, ldc=0, ldp=ptr(ldp), istar=ptr(istar)), (ldp, istar)
 
     @staticmethod
     def _broadcast_ld(ldp, istar, npv, npb):
@@ -309,10 +316,11 @@ class RoadRunnerModel(TransitModel):
             raise ValueError(f"t0 has {t0.shape[1]} columns but the epoch indices go up to {self.epids.max()}")
         if not isscalar(ldc) and not isinstance(ldc, torch.Tensor):
             ldc = np.atleast_1d(ldc)
-        ldp, istar = self._ld_profiles(ldc, npv, npb, device)
+        ld, _ld_keep = self._ld_descriptor(ldc, npv, npb, device)
         tab, _keep = self._table(device)
         d = self._device_data(device)
-        lc = RRLcData(npt=self.npt, nlc=self.nlc, npb=npb, nep=t0.shape[1], times=ptr(d['times']),
+        lc = RRLcData(npt=self.npt, nlc=self.nlc, npb=npb, nep=t0.shape[1], sorted=int(d['sorted']),
+                      uniform_nsamples=d['uniform_ns'], times=ptr(d['times']),
                       lcids=ptr(d['lcids']), pbids=ptr(d['pbids']), epids=ptr(d['epids']),
                       nsamples=ptr(d['nsamples']), exptimes=ptr(d['exptimes']))
         prm = RRParams(npv=npv, kcols=k.shape[1], t0cols=t0.shape[1], pad_=0, k=ptr(k), t0=ptr(t0), p=ptr(p),
@@ -320,15 +328,14 @@ class RoadRunnerModel(TransitModel):
         prec = _PRECISIONS[self.precision]
         dtype = torch.float64 if prec == _lib.RR_FP64 else torch.float32
         flux = torch.empty((npv, self.npt), dtype=dtype, device=device)
-        nbytes = lib.rr_workspace_bytes(npv, npb, self.ng, self.nlc)
+        nbytes = lib.rr_workspace_bytes(npv, self.npt, npb, self.ng, self.nlc)
         ws = self._workspace(device, nbytes)
         stream = stream_handle(device)
         byref = _lib.ctypes.byref
-        check(lib.rr_setup(prec, byref(tab), byref(lc), byref(prm), ptr(ldp), ptr(ws), ws.numel(), stream))
+        check(lib.rr_setup(prec, byref(tab), byref(lc), byref(prm), byref(ld), ptr(ws), ws.numel(), stream))
         if flux_events is not None:
             flux_events[0].record()
-        check(lib.rr_flux(prec, byref(tab), byref(lc), byref(prm), ptr(istar), ptr(ws), ws.numel(), ptr(flux),
-                          stream))
+        check(lib.rr_flux(prec, byref(tab), byref(lc), byref(prm), ptr(ws), ws.numel(), ptr(flux), stream))
         if flux_events is not None:
             flux_events[1].record()
         return flux

@@ -113,6 +113,8 @@ class TransitModel:
             if (nsamples < 1).any():
                 raise ValueError('nsamples must be >= 1')
             d = dict(
+                sorted=bool(self.npt < 2 or np.all(np.diff(self.time) >= 0)),
+                uniform_ns=int(nsamples[0]) if (nsamples == nsamples[0]).all() else 0,
                 times=torch.as_tensor(np.ascontiguousarray(self.time), device=device),
                 lcids=torch.as_tensor(self.lcids.astype(np.int32), device=device),
                 pbids=torch.as_tensor(self.pbids.astype(np.int32), device=device),

@@ -205,3 +205,35 @@ def test_empty_and_device_output(gpu):
     out = tm.evaluate(np.full(4, 0.1), np.tile([[[0.3, 0.1]]], (4, 1, 1)), np.zeros(4), np.full(4, 3.0),
                       np.full(4, 8.0), np.full(4, 1.5), copy=False)
     assert isinstance(out, torch.Tensor) and out.is_cuda and out.shape == (4, 64)
+
+
+def test_unsorted_times(gpu, oracle):
+    say('case: rr_small with shuffled times (general flux path)')
+    wl = workloads.rr_small(npv=16)
+    perm = np.random.default_rng(3).permutation(wl.times.size)
+    wl.times = wl.times[perm]
+    fo = oracle_flux(oracle, wl, 'quadratic')
+    for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
+        fg = gpu_flux(wl, prec)
+        assert np.nanmax(np.abs(fg - fo)) <= tol, prec
+
+
+def test_npt_not_multiple_of_run(gpu, oracle):
+    say('case: npt = 1001 (ragged tail, unaligned rows)')
+    wl = workloads.rr_small(npv=7, npt=1001)
+    fo = oracle_flux(oracle, wl, 'quadratic')
+    for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
+        fg = gpu_flux(wl, prec)
+        assert np.nanmax(np.abs(fg - fo)) <= tol, prec
+
+
+def test_all_points_in_transit_overflow_path(gpu, oracle):
+    say('case: 64 pv x 2000 times all in transit (queue overflow computed in place)')
+    wl = workloads.rr_small(npv=64, npt=2000)
+    wl.times = np.linspace(-0.02, 0.02, 2000)
+    wl.nsamples, wl.exptimes = 3, 0.005
+    fo = oracle_flux(oracle, wl, 'quadratic')
+    assert (fo < 1).mean() > 0.9
+    for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
+        fg = gpu_flux(wl, prec)
+        assert np.nanmax(np.abs(fg - fo)) <= tol, prec
