@@ -161,16 +161,16 @@ int rr_orbit(int npv, const double* p, const double* a, const double* inc, const
              const double* w, const double* k0, double* coeffs, double* win, void* stream);
 
 /* Transmission-spectroscopy model (TSModel): k is [npv, npb] (kcols = npb), t0 is
- * [npv, 1], nsamples[0]/exptimes[0] apply to all points, lcids/pbids ignored (Q16).
- * tab->weights == NULL selects exact per-pv 2-D weights at mean(k) (the default
- * precompute_weights=False path).  flux is [npv, npb, npt].  With RR_MIXED the
- * weights x profile contraction runs on FP32 MFMA.
+ * [npv] (t0cols = 1), nsamples[0] and exptimes[0] apply to every point and
+ * lcids/pbids are ignored (Q16).  tab->weights == NULL selects exact per-pv 2-D
+ * weights at mean(k) (the default precompute_weights=False path); otherwise the
+ * 3-D table is lerped at mean(k).  flux is [npv, npb, npt].  With RR_MIXED the
+ * weights x profile contraction runs on FP32 MFMA (v_mfma_f32_16x16x4f32).
  * Replaces tsmodel_serial (pytransit/models/roadrunner/model_trspec.py:11-93) as
  * called by TransmissionSpectroscopyModel.evaluate (tsmodel.py:46-130). */
 size_t ts_workspace_bytes(int npv, int npb, int ng, int nz, int precision);
 int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
-                const double* ldp, const double* istar, void* workspace, size_t workspace_bytes,
-                void* flux, void* stream);
+                const rr_ld* ld, void* workspace, size_t workspace_bytes, void* flux, void* stream);
 
 #ifdef __cplusplus
 }

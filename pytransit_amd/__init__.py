@@ -6,7 +6,8 @@ Public API mirrors ``pytransit`` (``pytransit/__init__.py:64-66``):
 """
 from .ldmodel import LDModel, TabulatedLDModel
 from .rrmodel import RoadRunnerModel, RRModel
+from .tsmodel import TransmissionSpectroscopyModel, TSModel
 
 __version__ = '0.1.0'
 
-__all__ = ['RoadRunnerModel', 'RRModel', 'LDModel', 'TabulatedLDModel']
+__all__ = ['RoadRunnerModel', 'RRModel', 'TransmissionSpectroscopyModel', 'TSModel', 'LDModel', 'TabulatedLDModel']

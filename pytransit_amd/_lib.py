@@ -71,6 +71,9 @@ _SIGNATURES = {
                         c_void_p, c_void_p]),
     'rr_evaluate': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd),
                             c_void_p, c_size_t, c_void_p, c_void_p]),
+    'ts_workspace_bytes': (c_size_t, [c_int, c_int, c_int, c_int, c_int]),
+    'ts_evaluate': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd),
+                            c_void_p, c_size_t, c_void_p, c_void_p]),
 }
 
 _lib = None

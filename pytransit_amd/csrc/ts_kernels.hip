@@ -1,0 +1,496 @@
+// Transmission-spectroscopy RoadRunner model (TSModel) on MI355X (gfx950).
+//
+// Replaces tsmodel_serial (pytransit/models/roadrunner/model_trspec.py:11-93):
+//   per pv: k_mean = mean(k), k_max = max(k), Taylor orbit, contact window at
+//   k_mean widened by 0.0015 + exptimes[0]; limb-darkening means
+//   ldm[pb, g] = W(k_mean)[g, :] . ldp[pb, :] with exact 2-D weights at k_mean
+//   (precompute_weights=False) or a k-lerp of the 3-D table; per (t, sample) one
+//   lens geometry at k_mean, fanned out over passbands with the area scaled by
+//   k^2/k_mean^2 (z <= 1 - k_max) or extrapolated linearly in k otherwise.
+//
+// Kernels:
+//   ts_setup_kernel    one 64-lane workgroup per pv (orbit on lanes 0-6, disk integrals)
+//   ts_weights_kernel  one lane per (pv, g): the 2-D weight row at k_mean, written
+//                      transposed W2T[pv][z][g] (FP64 and FP32, g padded to 16)
+//   ts_ldm_mfma_kernel FP32 MFMA contraction ldm = ldp . W2T, one workgroup per pv:
+//                      v_mfma_f32_16x16x4f32 tiles, W2T in LDS, profile rows streamed
+//   ts_ldm_f64_kernel  the FP64 contraction on the vector ALUs
+//   ts_flux_kernel     one lane per time, a workgroup per (pv, 256 times, 64|32
+//                      passbands): geometry in FP64, passband fan-out from LDS
+//
+// Flux layout [npv, npb, npt]; for one passband row 256 lanes write 1 KB (FP32)
+// contiguously.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <type_traits>
+
+#include "rr_device.hpp"
+#include "rr_internal.hpp"
+
+namespace rr {
+
+struct TsRec {
+    double c[10];
+    double bt1, bt4;     // window incl. the 0.0015 + exptimes[0] margin
+    double p, inv_p;
+    double kmean, kmax;
+    double ak;           // table mode: k-lerp weight
+    int32_t ik;          // table mode: row index, -1 for exact 2-D weights
+    int32_t good;
+    int32_t pad[2];
+};
+
+struct TsArgs {
+    const double* times; const double* k; const double* t0;
+    const double* exptimes; const int32_t* nsamples;   // element 0 applies to every point (Q16)
+    const double* p; const double* a; const double* inc; const double* e; const double* w;
+    int law, nldc; const double* ldc; const double* ldp_in; const double* istar_in;
+    const double* W; const double* ze; const double* mu;
+    TsRec* rec; double* istar; double* w2_64; float* w2_32; double* ldm64; float* ldm32;
+    int npv, npb, npt, nk, ng, ngp, nz, nzp, use_table;
+    double kmin, kmax_tab, dk, dg;
+};
+
+constexpr int kTsGroup = 8;
+
+// ---- per-pv setup; model_trspec.py:36-68 ----------------------------------------------
+__global__ void __launch_bounds__(64) ts_setup_kernel(TsArgs s) {
+    __shared__ double s_x[8][8], s_y[8][8], s_c[8][10], s_w[8][2], s_km[8];
+    const int sub = threadIdx.x / kTsGroup, j = threadIdx.x % kTsGroup;
+    const int pv = blockIdx.x * kTsGroup + sub;
+    const bool inrange = pv < s.npv;
+    const int q = inrange ? pv : 0;
+    const double a = s.a[q], e = s.e[q];
+    const bool valid = inrange && !(isnan(a) || a <= 1.0 || e < 0.0);
+    if (inrange && j == 0) {
+        // mean as numba's left-to-right sum; max of the row
+        double sum = 0.0, kmx = -INFINITY;
+        const double* kr = s.k + (size_t)q * s.npb;
+        for (int i = 0; i < s.npb; ++i) {
+            sum += kr[i];
+            kmx = fmax(kmx, kr[i]);
+        }
+        s_km[sub] = sum / s.npb;
+        TsRec* r = s.rec + q;
+        r->kmean = sum / s.npb;
+        r->kmax = kmx;
+        r->p = s.p[q];
+        r->inv_p = 1.0 / s.p[q];
+        r->good = valid ? 1 : 0;
+        // the reference tests kmin <= kmean <= max(k) (its kmax is shadowed by max(k))
+        const double km = sum / s.npb;
+        if (s.use_table && s.kmin <= km && km <= kmx) {
+            int ik = (int)floor((km - s.kmin) / s.dk);
+            ik = ik > s.nk - 2 ? s.nk - 2 : ik;
+            r->ik = ik;
+            r->ak = (km - s.kmin - ik * s.dk) / s.dk;
+        } else {
+            r->ik = -1;
+            r->ak = 0.0;
+        }
+    }
+    __syncthreads();
+    if (valid && j < 7) {
+        double x, y;
+        kepler_xy((j - 3) * 2e-2, s.p[q], a, cos(s.inc[q]), e, s.w[q], x, y);
+        s_x[sub][j] = x;
+        s_y[sub][j] = y;
+    }
+    __syncthreads();
+    if (valid && j == 0) taylor_from_stencil(s_x[sub], s_y[sub], s_c[sub]);
+    __syncthreads();
+    if (valid && j < 2) s_w[sub][j] = find_contact_point(s_km[sub], j == 0 ? 1 : 4, s_c[sub]);
+    __syncthreads();
+    if (valid) {
+        TsRec* r = s.rec + q;
+        r->c[j] = s_c[sub][j];
+        if (j < 2) r->c[8 + j] = s_c[sub][8 + j];
+        if (j == 0) r->bt1 = s_w[sub][0] - (0.0015 + s.exptimes[0]);
+        if (j == 1) r->bt4 = s_w[sub][1] + (0.0015 + s.exptimes[0]);
+    }
+}
+
+// Disk integrals per (pv, pb): given, analytic, or the 200-node trapezoid.
+__global__ void __launch_bounds__(256) ts_istar_kernel(TsArgs s) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= s.npv * s.npb) return;
+    double is;
+    if (s.law == RR_LD_TABULATED) {
+        is = s.istar_in[j];
+    } else {
+        const double* c = s.ldc + (size_t)j * s.nldc;
+        if (!ld_integral(s.law, c, is)) {
+            double sum = 0.0;
+            for (int m = 1; m < 200; ++m) sum += ld_trapz_segment(s.law, c, s.nldc, m);
+            is = 2.0 * kPi * sum;
+        }
+    }
+    s.istar[j] = is;
+}
+
+// ---- weights at k_mean, transposed: W2T[pv][z][g] --------------------------------------
+__global__ void __launch_bounds__(256) ts_weights_kernel(TsArgs s) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)s.npv * s.ngp) return;
+    const int pv = (int)(idx / s.ngp), ig = (int)(idx - (long long)pv * s.ngp);
+    const size_t base = (size_t)pv * s.nzp * s.ngp + ig;
+    const TsRec& r = s.rec[pv];
+    if (ig >= s.ng) {
+        for (int iz = 0; iz < s.nzp; ++iz) {
+            if (s.w2_64) s.w2_64[base + (size_t)iz * s.ngp] = 0.0;
+            if (s.w2_32) s.w2_32[base + (size_t)iz * s.ngp] = 0.0f;
+        }
+        return;
+    }
+    const double km = r.kmean;
+    if (r.ik >= 0) {
+        // (1-ak) W[ik] + ak W[ik+1] (device table layout [nk][nz][ng])
+        const double* w0 = s.W + (size_t)r.ik * s.nz * s.ng + ig;
+        const double* w1 = w0 + (size_t)s.nz * s.ng;
+        for (int iz = 0; iz < s.nz; ++iz) {
+            const double v = (1.0 - r.ak) * w0[(size_t)iz * s.ng] + r.ak * w1[(size_t)iz * s.ng];
+            if (s.w2_64) s.w2_64[base + (size_t)iz * s.ngp] = v;
+            if (s.w2_32) s.w2_32[base + (size_t)iz * s.ngp] = (float)v;
+        }
+    } else {
+        // calculate_weights_2d(k_mean); common.py:152-185
+        const double g = linspace_at(0.0, 1.0 - 1e-7, s.ng, ig);
+        const double b = g * (1.0 + km);
+        double a0 = cci_area(s.ze[0], km, b), sum = a0;
+        for (int iz = 1; iz < s.nz; ++iz) {
+            const double a1 = cci_area(s.ze[iz], km, b);
+            sum += a1 - a0;
+            a0 = a1;
+        }
+        a0 = cci_area(s.ze[0], km, b);
+        for (int iz = 0; iz < s.nz; ++iz) {
+            double v;
+            if (iz == 0) {
+                v = a0 / sum;
+            } else {
+                const double a1 = cci_area(s.ze[iz], km, b);
+                v = (a1 - a0) / sum;
+                a0 = a1;
+            }
+            if (s.w2_64) s.w2_64[base + (size_t)iz * s.ngp] = v;
+            if (s.w2_32) s.w2_32[base + (size_t)iz * s.ngp] = (float)v;
+        }
+    }
+    for (int iz = s.nz; iz < s.nzp; ++iz) {
+        if (s.w2_64) s.w2_64[base + (size_t)iz * s.ngp] = 0.0;
+        if (s.w2_32) s.w2_32[base + (size_t)iz * s.ngp] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ double ts_profile(const TsArgs& s, int pv, int pb, int iz) {
+    const size_t j = (size_t)pv * s.npb + pb;
+    if (s.law == RR_LD_TABULATED) return s.ldp_in[j * s.nz + iz];
+    return ld_law(s.law, s.mu[iz], s.ldc + j * s.nldc, s.nldc);
+}
+
+// ---- FP32 MFMA contraction: ldm[pv][pb][g] = sum_z ldp[pv][pb][z] * W2T[pv][z][g] ----
+// D(16x16) += A(16x4) B(4x16) per v_mfma_f32_16x16x4f32.  Lane l supplies
+// A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+r][l%16], r = 0..3.
+// A rows are passbands, B columns are g.  W2T of the pv sits in LDS with a padded
+// row stride so the 16 lanes reading one k hit distinct banks.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+constexpr int kMfmaK = 128;   // k per register-resident A fragment
+
+__global__ void __launch_bounds__(256) ts_ldm_mfma_kernel(TsArgs s) {
+    extern __shared__ float lds_b[];  // [nzp][ngp + 1]
+    const int pv = blockIdx.x;
+    const int ldb = s.ngp + 1;
+    for (int i = threadIdx.x; i < s.nzp * s.ngp; i += blockDim.x) {
+        const int iz = i / s.ngp, ig = i - iz * s.ngp;
+        lds_b[iz * ldb + ig] = s.w2_32[(size_t)pv * s.nzp * s.ngp + i];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int nrt = (s.npb + 15) / 16, nct = s.ngp / 16;
+    for (int rt = wave; rt < nrt; rt += 4) {
+        const int pb = rt * 16 + li;
+        const bool row_ok = pb < s.npb;
+        for (int ct0 = 0; ct0 < nct; ct0 += 8) {
+            f32x4_t acc[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) acc[c] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < s.nzp; k0 += kMfmaK) {
+                float afrag[kMfmaK / 4];
+#pragma unroll
+                for (int st = 0; st < kMfmaK / 4; ++st) {
+                    const int iz = k0 + st * 4 + lk;
+                    afrag[st] = (row_ok && iz < s.nz) ? (float)ts_profile(s, pv, pb, iz) : 0.0f;
+                }
+#pragma unroll
+                for (int st = 0; st < kMfmaK / 4; ++st) {
+                    const int iz = k0 + st * 4 + lk;
+                    if (k0 + st * 4 >= s.nzp) break;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) {
+                        const int ct = ct0 + c;
+                        if (ct < nct) {
+                            const float b = lds_b[iz * ldb + ct * 16 + li];
+                            acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[st], b, acc[c], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int ct = ct0 + c;
+                if (ct >= nct) break;
+                const int g = ct * 16 + li;
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int pbo = rt * 16 + lk * 4 + rr;
+                    if (pbo < s.npb && g < s.ng) s.ldm32[((size_t)pv * s.npb + pbo) * s.ng + g] = acc[c][rr];
+                }
+            }
+        }
+    }
+}
+
+// ---- FP64 contraction on the vector ALUs: one lane per (pv, pb, g) --------------------
+__global__ void __launch_bounds__(256) ts_ldm_f64_kernel(TsArgs s) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)s.npv * s.npb * s.ng) return;
+    const int ig = (int)(idx % s.ng);
+    const long long j = idx / s.ng;
+    const int pv = (int)(j / s.npb), pb = (int)(j - (long long)pv * s.npb);
+    const double* w = s.w2_64 + (size_t)pv * s.nzp * s.ngp + ig;
+    double d = 0.0;
+    for (int iz = 0; iz < s.nz; ++iz) d += w[(size_t)iz * s.ngp] * ts_profile(s, pv, pb, iz);
+    s.ldm64[idx] = d;
+}
+
+// ---- flux --------------------------------------------------------------------------
+constexpr int kTsThreads = 256;
+
+template <bool kMixed>
+struct TsCfg {
+    static constexpr int kPbChunk = kMixed ? 64 : 32;
+};
+
+template <bool kMixed>
+__global__ void __launch_bounds__(kTsThreads) ts_flux_kernel(TsArgs s, void* flux, int ntiles, int nchunks) {
+    using Real = typename std::conditional<kMixed, float, double>::type;
+    constexpr int PC = TsCfg<kMixed>::kPbChunk;
+    extern __shared__ __align__(16) unsigned char lds[];
+    Real* s_ldm = (Real*)lds;                       // [PC][ng]
+    Real* s_afac = s_ldm + (size_t)PC * s.ng;       // k^2 / k_mean^2
+    Real* s_kd = s_afac + PC;                       // k - k_mean
+    double* s_ist = (double*)(s_kd + PC + (kMixed ? (PC & 1) : 0));
+    __shared__ int s_any;
+
+    const int b = blockIdx.x;
+    const int chunk = b % nchunks;
+    const int tile = (b / nchunks) % ntiles;
+    const int pv = b / (nchunks * ntiles);
+    const int pb0 = chunk * PC;
+    const int npc = min(PC, s.npb - pb0);
+    const int t = tile * kTsThreads + threadIdx.x;
+    const bool tok = t < s.npt;
+    const TsRec& r = s.rec[pv];
+    Real* out = (Real*)flux + ((size_t)pv * s.npb + pb0) * s.npt;
+    if (!r.good) {
+        if (tok)
+            for (int q = 0; q < npc; ++q) out[(size_t)q * s.npt + t] = Real(NAN);
+        return;
+    }
+    double tc = 0.0;
+    bool inw = false;
+    if (tok) {
+#pragma clang fp contract(off)
+        const double t0v = s.t0[pv];
+        const double epoch = floor((s.times[t] - t0v + 0.5 * r.p) * r.inv_p);
+        const double u = s.times[t] - t0v + 0.5 * r.p;
+        const double rem = u - epoch * r.p;
+        const double ep = epoch + ((rem < 0.0) ? -1.0 : ((rem >= r.p) ? 1.0 : 0.0));
+        tc = s.times[t] - (t0v + ep * r.p);
+        inw = (r.bt1 <= tc) && (tc <= r.bt4);
+    }
+    if (threadIdx.x == 0) s_any = 0;
+    __syncthreads();
+    if (inw) s_any = 1;
+    __syncthreads();
+    if (!s_any) {
+        if (tok)
+            for (int q = 0; q < npc; ++q) out[(size_t)q * s.npt + t] = Real(1);
+        return;
+    }
+    // stage the chunk's limb-darkening means and per-passband factors
+    const double km = r.kmean;
+    for (int i = threadIdx.x; i < npc * s.ng; i += kTsThreads) {
+        const size_t src = ((size_t)pv * s.npb + pb0) * s.ng + i;
+        s_ldm[i] = kMixed ? (Real)s.ldm32[src] : (Real)s.ldm64[src];
+    }
+    for (int q = threadIdx.x; q < npc; q += kTsThreads) {
+        const double kq = s.k[(size_t)pv * s.npb + pb0 + q];
+        s_afac[q] = (Real)(kq * kq / (km * km));
+        s_kd[q] = (Real)(kq - km);
+        s_ist[q] = s.istar[(size_t)pv * s.npb + pb0 + q];
+    }
+    __syncthreads();
+
+    Real acc[PC];
+#pragma unroll
+    for (int q = 0; q < PC; ++q) acc[q] = Real(0);
+    const int ns = s.nsamples[0];
+    const double ext = s.exptimes[0];
+    if (inw) {
+        for (int is = 1; is <= ns; ++is) {
+            // geometry in FP64 once per (time, sample); model_trspec.py:79-84
+            const double off = ext * ((is - 0.5) / ns - 0.5);
+            const double z = taylor_z<double>(tc + off, r.c);
+            double ap0, kap;
+            ccia_kite<double>(1.0, km, z, ap0, kap);
+            const double dadk = 2.0 * km * kap;
+            const bool inside = z <= 1.0 - r.kmax;
+            const double g = z / (1.0 + km);
+            // lerp position (common.py:225-233, index clamped to ng-2)
+            bool gz = g > 1.0;
+            int ig = 0;
+            double al = 0.0;
+            if (!gz) {
+                ig = (int)floor(g / s.dg);
+                ig = ig > s.ng - 2 ? s.ng - 2 : (ig < 0 ? 0 : ig);
+                al = (g - ig * s.dg) / s.dg;
+            }
+            const Real ra = (Real)al, rb = (Real)(1.0 - al);
+            const Real rap0 = (Real)ap0, rdadk = (Real)dadk;
+#pragma unroll
+            for (int q = 0; q < PC; ++q) {
+                if (q < npc) {
+                    const Real* l = s_ldm + (size_t)q * s.ng;
+                    const Real ip = gz ? Real(0) : (rb * l[ig] + ra * l[ig + 1]);
+                    const Real area = inside ? rap0 * s_afac[q] : rap0 + s_kd[q] * rdadk;
+                    if (kMixed) acc[q] += ip * area;                                   // depth
+                    else acc[q] += (s_ist[q] - ip * area) / s_ist[q];                   // reference form
+                }
+            }
+        }
+    }
+    if (tok) {
+#pragma unroll
+        for (int q = 0; q < PC; ++q) {
+            if (q < npc) {
+                Real v;
+                if (!inw) v = Real(1);
+                else if (kMixed) v = Real(1.0f - (float)acc[q] * (float)(1.0 / (s_ist[q] * ns)));
+                else v = Real(acc[q] / ns);
+                out[(size_t)q * s.npt + t] = v;
+            }
+        }
+    }
+}
+
+}  // namespace rr
+
+using namespace rr;
+
+extern "C" {
+
+static size_t ts_sizes(int npv, int npb, int ng, int nz, int precision, size_t* off) {
+    const int ngp = (ng + 15) / 16 * 16, nzp = (nz + 3) / 4 * 4;
+    size_t b = 0;
+    off[0] = b; b += align_up((size_t)npv * sizeof(TsRec), 256);
+    off[1] = b; b += align_up((size_t)npv * npb * sizeof(double), 256);
+    const size_t w2e = (size_t)npv * nzp * ngp;
+    off[2] = b; b += align_up(w2e * (precision == RR_MIXED ? sizeof(float) : sizeof(double)), 256);
+    off[3] = b; b += align_up((size_t)npv * npb * ng * (precision == RR_MIXED ? sizeof(float) : sizeof(double)), 256);
+    return b;
+}
+
+size_t ts_workspace_bytes(int npv, int npb, int ng, int nz, int precision) {
+    if (npv < 0 || npb < 0 || ng < 0 || nz < 0) return 0;
+    size_t off[4];
+    return ts_sizes(npv, npb, ng, nz, precision, off);
+}
+
+int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const rr_ld* ld,
+                void* workspace, size_t workspace_bytes, void* flux, void* stream) {
+    if (!tab || !lc || !pv || !ld) return set_error(RR_EINVAL, "ts_evaluate: null descriptor");
+    if (precision != RR_FP64 && precision != RR_MIXED) return set_error(RR_EINVAL, "ts_evaluate: bad precision");
+    const int npv = pv->npv, npt = lc->npt, npb = lc->npb;
+    if (npv < 0 || npt < 0 || npb < 1) return set_error(RR_EINVAL, "ts_evaluate: bad sizes");
+    if (pv->kcols != npb)
+        return set_error(RR_EINVAL, "The transmission spectrum transit model requires that the number or radius "
+                                    "ratios and the number of passbands match.");
+    if (tab->ng < 2 || tab->nz < 1 || tab->nz > 1024 || tab->ng > 2048 || !tab->ze || !tab->mu)
+        return set_error(RR_EINVAL, "ts_evaluate: bad z grid / ng");
+    if (npv == 0 || npt == 0) return RR_OK;
+    if (!lc->times || !lc->nsamples || !lc->exptimes) return set_error(RR_EINVAL, "ts_evaluate: null time arrays");
+    if (!pv->k || !pv->t0 || !pv->p || !pv->a || !pv->inc || !pv->e || !pv->w || !flux)
+        return set_error(RR_EINVAL, "ts_evaluate: null parameter array");
+    if (ld->law == RR_LD_TABULATED) {
+        if (!ld->ldp || !ld->istar) return set_error(RR_EINVAL, "ts_evaluate: tabulated LD needs ldp and istar");
+    } else if (ld->law < RR_LD_UNIFORM || ld->law > RR_LD_POWER_2_PM || (ld->nldc > 0 && !ld->ldc)) {
+        return set_error(RR_EINVAL, "ts_evaluate: bad limb-darkening descriptor");
+    }
+    size_t off[4];
+    const size_t need = ts_sizes(npv, npb, tab->ng, tab->nz, precision, off);
+    if (!workspace || workspace_bytes < need)
+        return set_error(RR_EINVAL, "ts_evaluate: workspace of %zu bytes needed, got %zu", need, workspace_bytes);
+    if ((long long)npv * npb * npt > (1LL << 40)) return set_error(RR_ERANGE, "ts_evaluate: problem too large");
+
+    char* ws = (char*)workspace;
+    const bool mixed = precision == RR_MIXED;
+    TsArgs a;
+    a.times = lc->times; a.k = pv->k; a.t0 = pv->t0;
+    a.exptimes = lc->exptimes; a.nsamples = lc->nsamples;
+    a.p = pv->p; a.a = pv->a; a.inc = pv->inc; a.e = pv->e; a.w = pv->w;
+    a.law = ld->law; a.nldc = ld->nldc; a.ldc = ld->ldc; a.ldp_in = ld->ldp; a.istar_in = ld->istar;
+    a.W = tab->weights; a.ze = tab->ze; a.mu = tab->mu;
+    a.rec = (TsRec*)(ws + off[0]);
+    a.istar = (double*)(ws + off[1]);
+    a.w2_64 = mixed ? nullptr : (double*)(ws + off[2]);
+    a.w2_32 = mixed ? (float*)(ws + off[2]) : nullptr;
+    a.ldm64 = mixed ? nullptr : (double*)(ws + off[3]);
+    a.ldm32 = mixed ? (float*)(ws + off[3]) : nullptr;
+    a.npv = npv; a.npb = npb; a.npt = npt; a.nk = tab->nk; a.ng = tab->ng; a.nz = tab->nz;
+    a.ngp = (tab->ng + 15) / 16 * 16; a.nzp = (tab->nz + 3) / 4 * 4;
+    a.use_table = tab->weights != nullptr && tab->nk >= 2;
+    a.kmin = tab->kmin; a.kmax_tab = tab->kmax; a.dk = tab->dk; a.dg = tab->dg;
+    hipStream_t st = (hipStream_t)stream;
+
+    hipLaunchKernelGGL(ts_setup_kernel, dim3((npv + kTsGroup - 1) / kTsGroup), dim3(64), 0, st, a);
+    int rc = check_launch("ts_setup_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(ts_istar_kernel, dim3((npv * npb + 255) / 256), dim3(256), 0, st, a);
+    if ((rc = check_launch("ts_istar_kernel"))) return rc;
+    const long long nw = (long long)npv * a.ngp;
+    hipLaunchKernelGGL(ts_weights_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, a);
+    if ((rc = check_launch("ts_weights_kernel"))) return rc;
+    if (mixed) {
+        const size_t lds = (size_t)a.nzp * (a.ngp + 1) * sizeof(float);
+        if (lds > 160 * 1024) return set_error(RR_ERANGE, "ts_evaluate: nz*ng too large for the MFMA kernel");
+        hipLaunchKernelGGL(ts_ldm_mfma_kernel, dim3(npv), dim3(256), lds, st, a);
+        if ((rc = check_launch("ts_ldm_mfma_kernel"))) return rc;
+    } else {
+        const long long n = (long long)npv * npb * a.ng;
+        hipLaunchKernelGGL(ts_ldm_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+        if ((rc = check_launch("ts_ldm_f64_kernel"))) return rc;
+    }
+    const int ntiles = (npt + kTsThreads - 1) / kTsThreads;
+    if (mixed) {
+        constexpr int PC = TsCfg<true>::kPbChunk;
+        const int nchunks = (npb + PC - 1) / PC;
+        const size_t lds = (size_t)PC * a.ng * 4 + 2 * PC * 4 + 8 + PC * 8;
+        const long long nb = (long long)npv * ntiles * nchunks;
+        if (nb > 0x7fffffffLL) return set_error(RR_ERANGE, "ts_evaluate: grid too large");
+        hipLaunchKernelGGL(ts_flux_kernel<true>, dim3((unsigned)nb), dim3(kTsThreads), lds, st, a, flux, ntiles, nchunks);
+    } else {
+        constexpr int PC = TsCfg<false>::kPbChunk;
+        const int nchunks = (npb + PC - 1) / PC;
+        const size_t lds = (size_t)PC * a.ng * 8 + 2 * PC * 8 + PC * 8;
+        const long long nb = (long long)npv * ntiles * nchunks;
+        if (nb > 0x7fffffffLL) return set_error(RR_ERANGE, "ts_evaluate: grid too large");
+        hipLaunchKernelGGL(ts_flux_kernel<false>, dim3((unsigned)nb), dim3(kTsThreads), lds, st, a, flux, ntiles, nchunks);
+    }
+    return check_launch("ts_flux_kernel");
+}
+
+}  // extern "C"

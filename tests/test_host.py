@@ -53,7 +53,8 @@ def test_c_abi_rejects_bad_arguments():
     assert rc == -1 and b'unknown law' in lib.rr_last_error()
     rc = lib.rr_ld_profiles(2, None, 1, 1, 1, None, 4, None, None, None)
     assert rc == -1 and b'coefficients' in lib.rr_last_error()
-    assert lib.rr_workspace_bytes(4, 1000, 1, 100, 1) >= 4 * 128 + 4 * 100 * 12 + 4000 * 8
+    assert lib.rr_workspace_bytes(4, 1000, 1, 100, 1) >= 4 * 128 + 4 * 100 * 12
+    assert lib.ts_workspace_bytes(4, 8, 100, 40, 1) >= 4 * 8 * 100 * 4 + 4 * 40 * 112 * 4
 
 
 # ---- set_data (transitmodel.py:56-125) ----------------------------------------------

@@ -1,0 +1,345 @@
+"""Pins of the FP64 oracle (oracle/rr_oracle.py) against the reference's own
+known-answer values, closed forms, brute force and limiting cases, chosen so
+that a dropped term, a wrong sign or a transposed index fails one of them.
+
+Reference KATs used (paths relative to the PyTransit checkout):
+  tests/test_uniform_model_nb.py:22-82  F_EDGE = 1 - A(1, 0.1, z)/pi at edge z
+  tests/test_ma_quadratic_nb.py:23-62   eval_quad_z_s values (7 decimals)
+  tests/test_limb_darkening.py:21-199   ldi_* vs quad, edge cases
+  tests/test_z.py:23-68                 circular z at 7 times (4 decimals)
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.integrate import quad
+
+K = 0.1
+F_EDGE = {0.0: 1 - K ** 2, K: 1 - K ** 2, 1 - K: 1 - K ** 2, 1.0: 0.9951061298, 1 + K: 1.0}
+
+
+# ---- radial grid (common.py:131-149) -------------------------------------------------
+
+def test_z_grid_closed_form(oracle):
+    ze, zm = oracle.create_z_grid(0.7, 20, 20)
+    np.testing.assert_allclose(ze[:3], [0.035, 0.07, 0.105], atol=1e-15)
+    assert ze[19] == pytest.approx(0.7, abs=1e-15) and ze[39] == 1.0 and zm[0] == 0.0
+    assert np.all(np.diff(ze) > 0)
+    mu_edges = np.sqrt(1 - ze[19:] ** 2)          # limb edges uniform in mu
+    np.testing.assert_allclose(np.diff(mu_edges), -mu_edges[0] / 20, rtol=1e-12)
+    np.testing.assert_allclose(zm[1:], 0.5 * (ze[:-1] + ze[1:]), rtol=0)
+
+
+# ---- circle-circle areas (common.py:36-73) -------------------------------------------
+
+def test_area_special_cases(oracle):
+    for f in (lambda r1, r2, b: oracle.cci_area(r1, r2, b), lambda r1, r2, b: oracle.ccia_kite(r1, r2, b)[0]):
+        assert f(1.0, 0.1, 1.2) == 0.0                       # disjoint
+        assert f(1.0, 0.1, 0.5) == pytest.approx(math.pi * 0.01, rel=1e-15)   # planet inside
+        assert f(0.3, 1.0, 0.2) == pytest.approx(math.pi * 0.09, rel=1e-15)   # small circle covered
+    assert oracle.ccia_kite(1.0, 0.1, 0.5)[1] == math.pi
+
+
+def test_kite_equals_acos_formula_on_lens_domain(oracle):
+    rng = np.random.default_rng(0)
+    r2 = rng.uniform(0.01, 0.5, 5000)
+    b = rng.uniform(1 - r2 + 1e-6, 1 + r2 - 1e-6)
+    a1 = oracle.cci_area(1.0, r2, b)
+    a2, _ = oracle.ccia_kite(1.0, r2, b)
+    assert np.abs(a1 - a2).max() < 1e-12
+
+
+def test_area_brute_force_grid(oracle):
+    # area of {x^2+y^2 < r1^2} and {(x-b)^2+y^2 < r2^2} on a fine grid
+    n = 4000
+    for r1, r2, b in ((1.0, 0.3, 1.1), (1.0, 0.1, 0.95), (0.6, 0.4, 0.5)):
+        x = np.linspace(-r1, r1, n)
+        X, Y = np.meshgrid(x, x)
+        inside = (X ** 2 + Y ** 2 < r1 ** 2) & ((X - b) ** 2 + Y ** 2 < r2 ** 2)
+        area_grid = inside.sum() * (x[1] - x[0]) ** 2
+        assert oracle.cci_area(r1, r2, b) == pytest.approx(area_grid, rel=2e-3)
+        assert oracle.ccia_kite(r1, r2, b)[0] == pytest.approx(area_grid, rel=2e-3)
+
+
+def test_uniform_edge_kat(oracle):
+    """test_uniform_model_nb.py:22-82: F = 1 - A/pi at the standard edges."""
+    for z, f in F_EDGE.items():
+        assert 1 - oracle.ccia_kite(1.0, K, z)[0] / math.pi == pytest.approx(f, abs=5e-11)
+
+
+def test_kite_kappa0_is_dA_dk(oracle):
+    """dA/dr2 = 2 r2 kappa0 (the TSModel extrapolation, model_trspec.py:82-86)."""
+    z = np.linspace(0.92, 1.08, 17)
+    h = 1e-6
+    dadk = (oracle.ccia_kite(1.0, K + h, z)[0] - oracle.ccia_kite(1.0, K - h, z)[0]) / (2 * h)
+    np.testing.assert_allclose(dadk, 2 * K * oracle.ccia_kite(1.0, K, z)[1], rtol=1e-6)
+
+
+# ---- weight tables (common.py:152-223) -----------------------------------------------
+
+def test_weights_invariants_and_dk(oracle):
+    ze, _ = oracle.create_z_grid(0.7, 20, 20)
+    dk, dg, W = oracle.weights_3d(256, 0.005, 0.5, ze, 100)
+    assert dk == (0.5 - 0.005) / 256                          # Q1, not /(nk-1)
+    assert dg == pytest.approx((1 - 1e-7) / 99, rel=1e-15)
+    np.testing.assert_allclose(W.sum(-1), 1.0, atol=1e-13)
+    assert W.min() >= -1e-15
+    # row ik of the 3-D table is the 2-D table at linspace(kmin, kmax, nk)[ik]
+    ks = np.linspace(0.005, 0.5, 256)
+    for ik in (0, 49, 255):
+        np.testing.assert_allclose(oracle.weights_2d(ks[ik], ze, 100)[2], W[ik], atol=1e-15)
+
+
+def test_weights_brute_force(oracle):
+    """Weight = area of (annulus iz) x (planet disk) / planet-disk area inside the star."""
+    ze, _ = oracle.create_z_grid(0.7, 6, 6)
+    k, g = 0.2, 0.6
+    b = g * (1 + k)
+    n = 3000
+    x = np.linspace(b - k, b + k, n)
+    y = np.linspace(-k, k, n)
+    X, Y = np.meshgrid(x, y)
+    R = np.hypot(X, Y)
+    disk = (X - b) ** 2 + Y ** 2 < k ** 2
+    edges = np.concatenate([[0.0], ze])
+    cnt = np.array([(disk & (R >= edges[i]) & (R < edges[i + 1])).sum() for i in range(ze.size)], float)
+    w_grid = cnt / cnt.sum()
+    w = oracle.weights_2d(k, ze, 100)[2]
+    ig = int(round(g / (1 - 1e-7) * 99))
+    g_node = np.linspace(0, 1 - 1e-7, 100)[ig]
+    assert abs(g_node - g) < 6e-3
+    w_node = oracle.weights_2d(k, ze, 100)[2][ig]
+    np.testing.assert_allclose(w_node, w_grid, atol=2e-2)
+    assert w.shape == (100, 12)
+
+
+def test_interp_ld_mean(oracle):
+    lda = np.linspace(1.0, 2.0, 11)
+    dg = 0.1
+    g = np.array([0.0, 0.05, 0.5, 0.95, 1.0, 1.2, -0.1])
+    v = oracle.interp_ld_mean(g, dg, lda)
+    np.testing.assert_allclose(v[:5], 1.0 + g[:5], atol=1e-12)
+    assert v[5] == 0.0 and np.isnan(v[6])
+
+
+# ---- limb darkening (numba/ldmodels.py; tests/test_limb_darkening.py) -------------------
+
+def numerical_ldi(f, c):
+    return 2 * math.pi * quad(lambda mu: float(f(np.array([mu]), c)[0]) * mu, 0, 1)[0]
+
+
+@pytest.mark.parametrize('c', [[0.0, 0.0], [0.3, 0.2], [0.5, 0.3], [0.8, 0.5]])
+def test_ldi_quadratic(oracle, c):
+    assert oracle.ldi_quadratic(c) == pytest.approx(numerical_ldi(oracle.ld_quadratic, c), rel=1e-10)
+
+
+@pytest.mark.parametrize('c', [[0.3, 0.7], [0.5, 1.0], [0.8, 0.5], [0.0, 1.0]])
+def test_ldi_power_2(oracle, c):
+    assert oracle.ldi_power_2(c) == pytest.approx(numerical_ldi(oracle.ld_power_2, c), rel=1e-10)
+
+
+@pytest.mark.parametrize('c', [[0.1, 0.1], [0.5, 0.3], [1.0, 0.5]])
+def test_ldi_quadratic_tri(oracle, c):
+    assert oracle.ldi_quadratic_tri(c) == pytest.approx(numerical_ldi(oracle.ld_quadratic_tri, c), rel=1e-10)
+
+
+def test_ldi_linear_replicates_reference_quirk(oracle):
+    """Q7: numba/ldmodels.py:38-39 is 2pi/6 (3 - 2u); the true integral is 2pi/6 (3 - u)."""
+    u = 0.6
+    assert oracle.ldi_linear([u]) == pytest.approx(2 * math.pi / 6 * (3 - 2 * u), rel=1e-15)
+    assert numerical_ldi(oracle.ld_linear, [u]) == pytest.approx(2 * math.pi / 6 * (3 - u), rel=1e-10)
+
+
+def test_ld_edge_cases(oracle):
+    mu1 = np.array([1.0])
+    for f, c in ((oracle.ld_linear, [0.5]), (oracle.ld_quadratic, [0.3, 0.2]), (oracle.ld_quadratic_tri, [0.5, 0.3]),
+                 (oracle.ld_power_2, [0.3, 0.7]), (oracle.ld_nonlinear, [0.1, 0.2, 0.1, 0.05])):
+        assert f(mu1, c)[0] == pytest.approx(1.0, abs=1e-15)
+    mu = np.linspace(0.01, 1, 50)
+    np.testing.assert_allclose(oracle.ld_quadratic(mu, [0.6, 0.0]), oracle.ld_linear(mu, [0.6]), rtol=1e-12)
+    assert oracle.ld_general(mu1, [0.5, 0.3])[0] == 0.0
+
+
+def test_trapezoid_istar_fallback(oracle):
+    """Q8: laws without ldi use 2 pi trapz(z I, z) on mu = linspace(1, 0, 200)."""
+    c = np.array([0.1, 0.2, 0.1, 0.05])
+    _, ist = oracle.evaluate_ld('nonlinear', np.array([1.0]), c)
+    mu = np.linspace(1, 0, 200)
+    z = np.sqrt(1 - mu ** 2)
+    y = z * oracle.ld_nonlinear(mu, c)
+    assert ist[0, 0] == pytest.approx(2 * math.pi * np.sum(np.diff(z) * (y[1:] + y[:-1]) / 2), rel=1e-14)
+    assert ist[0, 0] == pytest.approx(numerical_ldi(oracle.ld_nonlinear, c), rel=2e-3)
+
+
+def test_integrate_profiles_matches_analytic(oracle):
+    ze, zm = oracle.create_z_grid(0.7, 64, 64)
+    mu = np.linspace(1, 0, 2000)
+    ldp = oracle.ld_quadratic(mu, [0.4, 0.2])[None, None, :]
+    assert oracle.integrate_profiles(mu, ldp)[0, 0] == pytest.approx(oracle.ldi_quadratic([0.4, 0.2]), rel=1e-5)
+
+
+# ---- orbit (orbits_py.py, taylor_z.py) ------------------------------------------------
+
+def test_circular_z_truth(oracle):
+    """tests/test_z.py:23-68: z at 7 times for p=1, a=3, i=pi/2, e=0, any w."""
+    times = np.array([0.0, 0.25, 0.4, 0.5, 0.6, 0.75 + 1e-8, 1.0])
+    z_truth = np.array([0., 3., -1.7634, -0., -1.7634, 3., 0.])
+    for w in (0.0, 0.5 * math.pi, math.pi, 1.5 * math.pi, 2 * math.pi):
+        z = []
+        for t in times:
+            f = oracle.ta_newton(t, 0.0, 1.0, 0.0, w)
+            zz = 3.0 * math.sqrt(1.0 - math.sin(w + f) ** 2 * math.sin(0.5 * math.pi) ** 2)
+            z.append(math.copysign(zz, math.sin(w + f)))
+        np.testing.assert_allclose(np.abs(z), np.abs(z_truth), atol=1e-4)
+
+
+def test_taylor_coefficients_vs_analytic_circular(oracle):
+    """e = 0: x = a sin(nt), y = -a cos(nt) cos i exactly; the 7-point stencils at
+    dt = 0.02 d reproduce the derivatives to the stencils' truncation order."""
+    for p, a, inc in ((1.0, 3.0, 0.5 * math.pi), (3.7, 9.0, 1.52), (12.0, 20.0, 1.565)):
+        c = oracle.taylor_coeffs(p, a, inc, 0.0, 0.0)
+        n = 2 * math.pi / p
+        exact = np.array([[0.0, a * n, 0.0, -a * n ** 3 / 6, 0.0],
+                          [-a * math.cos(inc), 0.0, a * n ** 2 * math.cos(inc) / 2, 0.0,
+                           -a * n ** 4 * math.cos(inc) / 24]])
+        wdt = n * 0.02
+        assert c[0, 1] == pytest.approx(exact[0, 1], rel=wdt ** 6)        # velocity: 6th order
+        assert c[0, 3] == pytest.approx(exact[0, 3], rel=wdt ** 4 * 2)    # jerk: 4th order
+        assert c[1, 0] == pytest.approx(exact[1, 0], rel=1e-14)
+        if math.cos(inc) > 1e-3:
+            assert c[1, 2] == pytest.approx(exact[1, 2], rel=wdt ** 6)
+        assert abs(c[0, 2]) < 1e-9 and abs(c[0, 4]) < 1e-6 and abs(c[1, 1]) < 1e-9
+
+
+def test_taylor_z_vs_exact_kepler_in_window(oracle):
+    """Within the transit window the 4th-order series follows the exact orbit."""
+    p, a, inc, e, w = 2.5, 7.0, 1.50, 0.3, 1.1
+    c = oracle.taylor_coeffs(p, a, inc, e, w)
+    t = np.linspace(-0.05, 0.05, 21)          # this orbit crosses the disk in ~0.1 d
+    exact = []
+    for tt in t:
+        f = oracle.ta_newton(tt, 0.0, p, e, w)
+        r = a * (1 - e ** 2) / (1 + e * math.cos(f))
+        exact.append(math.hypot(-r * math.cos(w + f), -r * math.sin(w + f) * math.cos(inc)))
+    err = np.abs(oracle.sep(t, c) - np.array(exact))
+    assert err.max() < 1e-4
+    assert err[np.abs(t) <= 0.02].max() < 1e-6          # truncation error grows as t^5
+
+
+def test_contact_times_vs_closed_form(oracle):
+    """e = 0: T14 = (p/pi) asin(sqrt((1+k)^2 - b^2) / (a sin i))."""
+    for p, a, b, k in ((1.0, 3.0, 0.0, 0.1), (4.2, 11.0, 0.5, 0.08), (9.0, 18.0, 0.8, 0.12)):
+        inc = math.acos(b / a)
+        c = oracle.taylor_coeffs(p, a, inc, 0.0, 0.0)
+        t1, t4 = oracle.bounding_box(k, c)
+        t14 = p / math.pi * math.asin(math.sqrt((1 + k) ** 2 - b ** 2) / (a * math.sin(inc)))
+        assert t4 - t1 == pytest.approx(t14, rel=2e-4)
+        assert t1 == pytest.approx(-t4, abs=1e-9)
+        assert float(oracle.sep(t4, c)) == pytest.approx(1 + k, abs=2e-5)
+
+
+# ---- flux -----------------------------------------------------------------------------
+
+def _single(oracle, law, times, k, ldc, p=3.0, a=8.0, inc=0.5 * math.pi, nss=1, ext=0.0, **kw):
+    om = oracle.OracleRR(law, **kw)
+    om.set_data(times, nsamples=nss, exptimes=ext)
+    return om, om.evaluate(np.array([[k]]), np.asarray(ldc, float).reshape(1, 1, -1), np.array([[0.0]]), p, a, inc)[0]
+
+
+def test_uniform_ld_flux_is_one_minus_area_over_pi(oracle):
+    """Weight rows sum to 1, so uniform LD gives I = 1 exactly and F = 1 - A(1,k,z)/pi."""
+    times = np.linspace(-0.12, 0.12, 241)
+    om, f = _single(oracle, 'uniform', times, K, [])
+    c = oracle.taylor_coeffs(3.0, 8.0, 0.5 * math.pi, 0.0, 0.0)
+    z = oracle.sep(times, c)
+    expect = 1 - oracle.ccia_kite(1.0, K, z)[0] / math.pi
+    t1, t4 = oracle.bounding_box(K, c)
+    expect[(times < t1 - 0.003) | (times > t4 + 0.003)] = 1.0
+    np.testing.assert_allclose(f, expect, atol=1e-14)
+    assert f.min() == pytest.approx(1 - K ** 2, abs=1e-14)
+
+
+def test_quadratic_rr_converges_to_mandel_agol(oracle):
+    """RR's discretised limb darkening converges to the analytic quadratic model
+    (pinned by test_ma_quadratic_nb.py) as the grids refine."""
+    u = [0.2, 0.1]
+    for z, f_ma in ((0.2, 0.98914137), (0.0, 0.98909638), (0.9, 0.99164875), (1.0, 0.99573423)):
+        assert oracle.eval_quad_z(z, K, u) == pytest.approx(f_ma, abs=1.5e-7)   # assert_almost_equal(7)
+    zs = np.linspace(0.0, 1.09, 200)
+    errs = []
+    for nz, ng in ((20, 100), (80, 400)):
+        ze, zm = oracle.create_z_grid(0.7, nz, nz)
+        mu = oracle.mu_nodes(zm)
+        _, dg, w2 = oracle.weights_2d(K, ze, ng)
+        ldm = w2 @ oracle.ld_quadratic(mu, u)
+        ist = oracle.ldi_quadratic(u)
+        f_rr = 1 - oracle.interp_ld_mean(zs / (1 + K), dg, ldm) * oracle.ccia_kite(1.0, K, zs)[0] / ist
+        f_ma = np.array([oracle.eval_quad_z(z, K, u) for z in zs])
+        errs.append(np.abs(f_rr - f_ma).max())
+    assert errs[0] < 1e-5 and errs[1] < errs[0] / 3
+
+
+def test_supersampling_reduces_to_midexposure(oracle):
+    times = np.linspace(-0.1, 0.1, 101)
+    _, f1 = _single(oracle, 'quadratic', times, K, [0.3, 0.1])
+    _, fn = _single(oracle, 'quadratic', times, K, [0.3, 0.1], nss=7, ext=0.0)
+    np.testing.assert_allclose(fn, f1, atol=1e-15)
+    _, fe = _single(oracle, 'quadratic', times, K, [0.3, 0.1], nss=2, ext=0.02)
+    _, fa = _single(oracle, 'quadratic', times - 0.005, K, [0.3, 0.1])
+    _, fb = _single(oracle, 'quadratic', times + 0.005, K, [0.3, 0.1])
+    inner = np.abs(times) < 0.05
+    np.testing.assert_allclose(fe[inner], 0.5 * (fa + fb)[inner], atol=1e-15)
+
+
+def test_phase_reduction_bjd_offset_invariance(oracle):
+    times = np.linspace(-0.1, 0.1, 201)
+    p = 3.0
+    om, f0 = _single(oracle, 'quadratic', times, K, [0.3, 0.1], p=p)
+    n = 819000
+    om2 = oracle.OracleRR('quadratic')
+    om2.set_data(times + n * p)
+    f1 = om2.evaluate(np.array([[K]]), np.array([[[0.3, 0.1]]]), np.array([[0.0]]), p, 8.0, 0.5 * math.pi)[0]
+    assert np.abs(f1 - f0).max() < 1e-9
+    assert (f0[np.abs(times) > 0.09] == 1.0).all()
+
+
+def test_tsmodel_equal_k_reduces_to_rr(oracle):
+    ze, zm = oracle.create_z_grid(0.7, 20, 20)
+    mu = oracle.mu_nodes(zm)
+    times = np.linspace(-0.12, 0.12, 301)
+    ldp, ist = oracle.evaluate_ld('quadratic', mu, np.array([[[0.3, 0.1], [0.3, 0.1], [0.3, 0.1]]]))
+    k = np.full((1, 3), 0.004)                     # outside klims: RR also uses exact 2-D weights
+    fts = oracle.tsmodel_serial(times, k, [0.0], [3.0], [8.0], [1.5], [0.0], [0.0], [1], [0.0], ldp, ist, None, None,
+                                0.005, 0.5, 100, None, ze)
+    om = oracle.OracleRR('quadratic')
+    om.set_data(times)
+    frr = om.evaluate(k[:, :1], np.array([[[0.3, 0.1]]]), np.array([[0.0]]), 3.0, 8.0, 1.5)
+    for ipb in range(3):
+        np.testing.assert_allclose(fts[0, ipb], frr[0], atol=1e-15)
+
+
+def test_reference_flux_fixture_structure(oracle):
+    """tests/reference_flux.npz (no generator recorded): symmetric, exactly 1
+    out of transit, and column 0 is a k = 0.1 near-uniform central transit."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'reference_flux.npz'))
+    f = d['flux']
+    assert f.shape == (2000, 4)
+    np.testing.assert_allclose(f, f[::-1], atol=1e-12)
+    assert (f[:200] == 1.0).all()
+    assert 1 - f[:, 0].min() == pytest.approx(K ** 2, abs=1e-5)
+    assert np.all(np.diff(1 - f.min(0)) > 0)       # depth grows with limb darkening
+
+
+def test_c_restatement_matches_numpy_oracle(oracle):
+    from oracle import rr_cpu
+    from pytransit_amd import workloads
+    wl = workloads.rr_bjd(npv=8, npt=4000)
+    om = oracle.OracleRR('power-2')
+    om.set_data(wl.times, nsamples=10, exptimes=wl.exptimes)
+    fo = om.evaluate(wl.k, wl.ldc, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w)
+    cm = rr_cpu.CpuRR('power-2')
+    cm.set_data(wl.times, nsamples=10, exptimes=wl.exptimes)
+    fc = cm.evaluate(wl.k, wl.ldc, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w, nthreads=2)
+    assert np.abs(fc - fo).max() < 1e-12
+    assert (fo < 1).any()

@@ -1,0 +1,90 @@
+"""GPU parity of the TSModel path (model_trspec.py:11-93, serial semantics) against
+the FP64 oracle.  Bars: FP64 <= 1e-9 absolute flux, mixed (FP32 MFMA contraction,
+FP32 passband fan-out) <= 1 ppm."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from pytransit_amd import TabulatedLDModel, TSModel, workloads
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-9
+MIXED_TOL = 1e-6
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def ts_case(oracle, npv=4, npt=1500, npb=64, seed=2):
+    ze, zm = oracle.create_z_grid(0.7, 64, 64)
+    mu = oracle.mu_nodes(zm)
+    wl = workloads.ts_many_pb(npv=npv, npt=npt, npb=npb, seed=seed, mu=mu)
+    return wl, ze, mu
+
+
+def oracle_ts(oracle, wl, ze, weights=None, dk=None, dg=None, kmin=0.005, kmax=0.5, nsamples=1, exptime=0.0):
+    return oracle.tsmodel_serial(wl.times, wl.k, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w, np.array([nsamples]),
+                                 np.array([exptime]), wl.ldp, wl.istar, weights, dk, kmin, kmax, 100, dg, ze)
+
+
+@pytest.mark.parametrize('prec,tol', [('fp64', FP64_TOL), ('mixed', MIXED_TOL)])
+def test_ts_tabulated_profiles(gpu, oracle, prec, tol):
+    say(f'case: TS 4 pv x 64 pb x 1500 t, 128 mu nodes, tabulated LD, {prec}')
+    wl, ze, mu = ts_case(oracle)
+    fo = oracle_ts(oracle, wl, ze)
+    tm = TSModel(TabulatedLDModel(wl.ldp, wl.istar), nzin=64, nzlimb=64, precision=prec)
+    tm.set_data(wl.times)
+    fg = np.asarray(tm.evaluate(wl.k, None, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w), float)
+    assert fg.shape == fo.shape == (4, 64, 1500)
+    err = np.nanmax(np.abs(fg - fo))
+    say(f'  max |dF| = {err:.3e}, in-transit points {(fo < 1).sum()}')
+    assert (fo < 1).sum() > 10000
+    assert err <= tol
+
+
+@pytest.mark.parametrize('prec,tol', [('fp64', FP64_TOL), ('mixed', MIXED_TOL)])
+def test_ts_named_law_supersampled_and_invalid(gpu, oracle, prec, tol):
+    say(f'case: TS quadratic law, nss=3, exposure 0.01 d, one invalid pv, {prec}')
+    wl, ze, mu = ts_case(oracle, npv=5, npt=800, npb=37)
+    wl.a = wl.a.copy()
+    wl.a[2] = 0.5
+    ldc = np.stack([wl.u1, wl.u2], -1)
+    ldp, istar = oracle.evaluate_ld('quadratic', mu, ldc)
+    wl.ldp, wl.istar = ldp, istar
+    fo = oracle_ts(oracle, wl, ze, nsamples=3, exptime=0.01)
+    tm = TSModel('quadratic', nzin=64, nzlimb=64, precision=prec)
+    tm.set_data(wl.times, nsamples=3, exptimes=0.01)
+    fg = np.asarray(tm.evaluate(wl.k, ldc, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w), float)
+    assert np.array_equal(np.isnan(fo), np.isnan(fg)) and np.isnan(fg[2]).all()
+    err = np.nanmax(np.abs(fg - fo))
+    say(f'  max |dF| = {err:.3e}')
+    assert err <= tol
+
+
+@pytest.mark.parametrize('prec,tol', [('fp64', FP64_TOL), ('mixed', MIXED_TOL)])
+def test_ts_precomputed_table(gpu, oracle, prec, tol):
+    say(f'case: TS with precompute_weights=True (3-D table lerp at mean k), {prec}')
+    wl, ze, mu = ts_case(oracle, npv=3, npt=600, npb=20)
+    dk, dg, W = oracle.weights_3d(256, 0.005, 0.5, ze, 100)
+    fo = oracle_ts(oracle, wl, ze, weights=W, dk=dk, dg=dg)
+    tm = TSModel(TabulatedLDModel(wl.ldp, wl.istar), nzin=64, nzlimb=64, precision=prec, precompute_weights=True)
+    tm.set_data(wl.times)
+    fg = np.asarray(tm.evaluate(wl.k, None, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w), float)
+    err = np.nanmax(np.abs(fg - fo))
+    say(f'  max |dF| = {err:.3e}')
+    assert err <= tol
+
+
+def test_ts_equal_k_reduces_to_rr(gpu, oracle):
+    say('case: TS with equal k in all passbands vs RR with exact 2-D weights')
+    wl, ze, mu = ts_case(oracle, npv=3, npt=700, npb=8)
+    wl.k = np.repeat(wl.k[:, :1], 8, axis=1)
+    tm = TSModel(TabulatedLDModel(wl.ldp, wl.istar), nzin=64, nzlimb=64, precision='fp64')
+    tm.set_data(wl.times)
+    fts = np.asarray(tm.evaluate(wl.k, None, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w), float)
+    fo = oracle_ts(oracle, wl, ze)
+    assert np.nanmax(np.abs(fts - fo)) <= FP64_TOL
