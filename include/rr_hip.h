@@ -172,6 +172,19 @@ size_t ts_workspace_bytes(int npv, int npb, int ng, int nz, int precision);
 int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
                 const rr_ld* ld, void* workspace, size_t workspace_bytes, void* flux, void* stream);
 
+/* Gaussian log-likelihood of the model for one light curve and its gradient,
+ * fused with the model so the flux is never written:
+ *   lnl[pv] = sum_t -ln sigma[pv] - 1/2 ln 2 pi - 1/2 ((obs_t - m_t) / sigma[pv])^2
+ *   grad[pv, :] = d lnl / d(k, t0, p, a, i, c0, c1)   (nullable)
+ * c0, c1 are the coefficients of the quadratic or power-2 law; k must lie in the
+ * weight table's [kmin, kmax] (else NaN); nlc = 1, kcols = 1.  The likelihood is
+ * lnlike_normal (pytransit/lpf/loglikelihood/wnloglikelihood.py:22-35) applied to
+ * rr_full's flux (model_full.py:9-100); the gradient has no counterpart in the
+ * reference (its dfdk/dfdb in common.py:104-128 are approximate and unused). */
+int rr_lnlike(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
+              const rr_ld* ld, const double* obs, const double* sigma, double* lnl, double* grad,
+              void* stream);
+
 #ifdef __cplusplus
 }
 #endif

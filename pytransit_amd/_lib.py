@@ -72,6 +72,8 @@ _SIGNATURES = {
     'rr_evaluate': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd),
                             c_void_p, c_size_t, c_void_p, c_void_p]),
     'ts_workspace_bytes': (c_size_t, [c_int, c_int, c_int, c_int, c_int]),
+    'rr_lnlike': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd), c_void_p,
+                          c_void_p, c_void_p, c_void_p, c_void_p]),
     'ts_evaluate': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd),
                             c_void_p, c_size_t, c_void_p, c_void_p]),
 }

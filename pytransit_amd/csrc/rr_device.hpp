@@ -51,9 +51,12 @@ __device__ __forceinline__ double cci_area(double r1, double r2, double b) {
 }
 
 // Lens area and kappa_0 of circles r1, r2 at separation b by the kite formula
-// (Agol et al. 2020); common.py:52-73 with tsort common.py:5-33.
+// (Agol et al. 2020); common.py:52-73 with tsort common.py:5-33.  akite is the
+// area of the kite spanned by the centres and the intersection points (0 outside
+// the lens case).
 template <typename T>
-__device__ __forceinline__ void ccia_kite(T r1, T r2, T b, T& area, T& kap0) {
+__device__ __forceinline__ void ccia_kite3(T r1, T r2, T b, T& area, T& kap0, T& akite_out) {
+    akite_out = T(0);
     if (r1 + r2 <= b) { area = T(0); kap0 = T(0); return; }
     if (d_abs(r1 - r2) < b && b <= r1 + r2) {
         // descending sort of (r1, r2, b): x >= y >= z
@@ -66,11 +69,18 @@ __device__ __forceinline__ void ccia_kite(T r1, T r2, T b, T& area, T& kap0) {
         const T k1 = d_atan2(T(2) * akite, (r1 - r2) * (r1 + r2) + b * b);
         area = r1 * r1 * k1 + r2 * r2 * k0 - akite;
         kap0 = k0;
+        akite_out = akite;
         return;
     }
     if (b <= r1 - r2) { area = pi_of<T>() * (r2 * r2); kap0 = pi_of<T>(); return; }
     if (b <= r2 - r1) { area = pi_of<T>() * (r1 * r1); kap0 = T(0); return; }
     area = T(NAN); kap0 = T(NAN);
+}
+
+template <typename T>
+__device__ __forceinline__ void ccia_kite(T r1, T r2, T b, T& area, T& kap0) {
+    T akite;
+    ccia_kite3<T>(r1, r2, b, area, kap0, akite);
 }
 
 // Mean limb darkening for grazing parameter g; common.py:225-233.  The interval

@@ -343,5 +343,59 @@ class RoadRunnerModel(TransitModel):
     def __call__(self, *args, **kwargs):
         return self.evaluate(*args, **kwargs)
 
+    # ---------------------------------------------------------------------------------
+    def lnlike(self, k, ldc, t0, p, a, i, obs, sigma, e=0.0, w=0.0, grad: bool = False, copy: bool = True):
+        """Gaussian log-likelihood of ``obs`` under the model, per parameter vector,
+        fused with the model on the GPU (no flux array is written):
+
+            lnL = sum_t -ln sigma - 1/2 ln(2 pi) - 1/2 ((obs_t - model_t) / sigma)^2
+
+        (the form of lnlike_normal, pytransit/lpf/loglikelihood/wnloglikelihood.py:
+        22-35).  With ``grad=True`` also returns d lnL / d(k, t0, p, a, i, c0, c1)
+        as ``[npv, 7]`` from the hand-written adjoint, where c0, c1 are the two
+        coefficients of the 'quadratic' or 'power-2' law.  One light curve, k of
+        shape (npv,) or (npv, 1) inside ``klims``; ``sigma`` is a scalar or one
+        value per pv.
+        """
+        lnl, g = self._lnlike_device(k, ldc, t0, p, a, i, obs, sigma, e, w, grad)
+        if copy:
+            return (lnl.cpu().numpy(), g.cpu().numpy()) if grad else lnl.cpu().numpy()
+        return (lnl, g) if grad else lnl
+
+    def _lnlike_device(self, k, ldc, t0, p, a, i, obs, sigma, e=0.0, w=0.0, grad=True):
+        lib = _lib.load()
+        device = self.device
+        if self.time is None:
+            raise ValueError('set_data() must be called before lnlike()')
+        if self.nlc != 1:
+            raise ValueError('lnlike supports one light curve')
+        if self.ldname not in ('quadratic', 'power-2'):
+            raise ValueError("lnlike supports the 'quadratic' and 'power-2' laws")
+        npv, k, t0, (p, a, inc, e, w) = self._params(k, t0, p, a, i, e, w, device)
+        if k.shape[1] != 1:
+            raise ValueError('lnlike needs one radius ratio per parameter vector')
+        ld, _keep = self._ld_descriptor(ldc, npv, 1, device)
+        obs = _as_dev(obs, device).reshape(-1)
+        if obs.numel() != self.npt:
+            raise ValueError(f'obs has {obs.numel()} values, the data have {self.npt} times')
+        sig = _as_dev(sigma, device).reshape(-1)
+        if sig.numel() not in (1, npv):
+            raise ValueError('sigma must be a scalar or one value per parameter vector')
+        sig = sig.expand(npv).contiguous()
+        tab, _k = self._table(device)
+        d = self._device_data(device)
+        lc = RRLcData(npt=self.npt, nlc=1, npb=1, nep=t0.shape[1], sorted=int(d['sorted']),
+                      uniform_nsamples=d['uniform_ns'], times=ptr(d['times']), lcids=ptr(d['lcids']),
+                      pbids=ptr(d['pbids']), epids=ptr(d['epids']), nsamples=ptr(d['nsamples']),
+                      exptimes=ptr(d['exptimes']))
+        prm = RRParams(npv=npv, kcols=1, t0cols=t0.shape[1], pad_=0, k=ptr(k), t0=ptr(t0), p=ptr(p), a=ptr(a),
+                       inc=ptr(inc), e=ptr(e), w=ptr(w))
+        lnl = torch.empty(npv, dtype=torch.float64, device=device)
+        g = torch.empty((npv, 7), dtype=torch.float64, device=device) if grad else None
+        byref = _lib.ctypes.byref
+        check(lib.rr_lnlike(_PRECISIONS[self.precision], byref(tab), byref(lc), byref(prm), byref(ld), ptr(obs),
+                            ptr(sig), ptr(lnl), ptr(g), stream_handle(device)))
+        return lnl, g
+
 
 RRModel = RoadRunnerModel
