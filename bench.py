@@ -19,9 +19,12 @@ Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
 import argparse
 import json
 import os
+import re
 import statistics
 import sys
 import time
+
+import numpy as np
 
 METRIC = "model points/s mixed vs FP64 plain at ≤1 ppm flux err; served p50/p99 lat"
 HBM_PEAK_GBS = 8000.0
@@ -34,7 +37,37 @@ def parse():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--cpu-reps', type=int, default=5)
+    ap.add_argument('--no-serve', action='store_true', help='skip the serving leg')
+    ap.add_argument('--serve-requests', type=int, default=300)
+    ap.add_argument('--serve-rate', type=float, default=2000.0, help='Poisson arrival rate, requests/s')
+    ap.add_argument('--copy-probe', action='store_true',
+                    help='also launch a plain 256 MiB device copy (the calibration kernel of the HBM counter runs)')
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                            'rr_flux_mixed_rr_bjd_hbm_traffic.md')
+
+
+def measured_traffic():
+    """HBM bytes per mixed flux launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            m = re.search(r'traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)', f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def copy_probe(torch, dev):
+    """A copy of known size: 256 MiB read + 256 MiB written per launch."""
+    n = 64 << 20
+    src = torch.ones(n, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
 
 
 def flux_kernel_bytes(npv, npt, ng, out_bytes):
@@ -59,6 +92,65 @@ def time_path(tm, args_dev, steps, warmup):
     wall = time.perf_counter() - t0
     kern_ms = statistics.mean(a.elapsed_time(b) for a, b in ev)
     return wall / steps, kern_ms, out
+
+
+def serve_stream(torch, dev, st, slots, max_batch, budget):
+    """Replay the request stream in real time through one server; returns the
+    per-request latencies (s), the points served and the wall time from the
+    first arrival to the last completion."""
+    from pytransit_amd.serving import Dataset, RRServer
+    dsets = [Dataset(**d) for d in st.datasets]
+    srv = RRServer(hbm_budget=budget, slots=slots, max_batch=max_batch, precision='mixed', device=dev,
+                   auto_release=True)
+    # warm-up: every light-curve set resident, kernels loaded
+    for j, d in enumerate(dsets):
+        rq = next((r for r in st.requests if r['dataset'] == j), None)
+        if rq is not None:
+            srv.submit(d, rq['k'][:8], rq['ldc'][:8], rq['t0'][:8], rq['p'][:8], rq['a'][:8], rq['i'][:8],
+                       ld=rq['ld'])
+    srv.run(timeout=120)
+    torch.cuda.synchronize()
+    srv.completed.clear()
+    reqs = st.requests
+    n = len(reqs)
+    t_start = time.perf_counter()
+    i = 0
+    while i < n or not srv.idle:
+        now = time.perf_counter() - t_start
+        while i < n and reqs[i]['arrival'] <= now:
+            rq = reqs[i]
+            srv.submit(dsets[rq['dataset']], rq['k'], rq['ldc'], rq['t0'], rq['p'], rq['a'], rq['i'], ld=rq['ld'],
+                       arrival=t_start + rq['arrival'])
+            i += 1
+        srv.step()
+    done = srv.completed
+    lat = np.array([r.latency for r in done])
+    wall = max(r.t_done for r in done) - t_start
+    points = sum(r.points for r in done)
+    return dict(latency=lat, points=points, wall=wall, batch=np.mean([r.batch_size for r in done]))
+
+
+def serving_leg(torch, dev, nreq, rate):
+    st = workloads_mod().serving_stream(n_requests=nreq, rate=rate)
+    budget = 48 << 30
+    out = {}
+    for name, slots, mb in (('batched', 4, 16), ('unbatched', 1, 1)):
+        r = serve_stream(torch, dev, st, slots, mb, budget)
+        out[name] = {"throughput_points_per_s": r['points'] / r['wall'], "requests_per_s": nreq / r['wall'],
+                     "p50_ms": float(np.percentile(r['latency'], 50) * 1e3),
+                     "p99_ms": float(np.percentile(r['latency'], 99) * 1e3),
+                     "wall_s": r['wall'], "mean_batch": float(r['batch']), "slots": slots, "max_batch": mb}
+    out["workload"] = (f"{nreq} fit requests, Poisson arrivals at {rate:g}/s; light-curve sets of log-uniform "
+                       f"1k-200k points in 1-64 passbands (24-set pool, resident), log-uniform 32-8192 pv, "
+                       f"quadratic/power-2/nonlinear-4 LD, nss in {{1,5,10}}; mixed precision; flux left on device")
+    out["points"] = int(sum(rq['k'].shape[0] * st.datasets[rq['dataset']]['time'].size for rq in st.requests))
+    out["hbm_budget_bytes"] = budget
+    return out
+
+
+def workloads_mod():
+    from pytransit_amd import workloads
+    return workloads
 
 
 def cpu_baseline(wl, reps):
@@ -101,11 +193,12 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path has no multi-GPU form "
               f"in this build.", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
     from pytransit_amd import RoadRunnerModel, workloads
 
     dev = torch.device('cuda', 0)
+    if args.copy_probe:
+        copy_probe(torch, dev)
     wl = workloads.rr_bjd()
     npv, npt = wl.k.shape[0], wl.times.size
     args_dev = tuple(torch.as_tensor(x, device=dev) for x in (wl.k, wl.ldc, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w))
@@ -126,7 +219,7 @@ def main():
     fp64_pps = npts / res['fp64']['step_s']
     kbytes = flux_kernel_bytes(npv, npt, res['mixed']['ng'], 4)
     achieved = kbytes / (res['mixed']['kern_ms'] * 1e-3) / 1e9
-    traffic = None
+    traffic = measured_traffic()
     line = {
         "metric": METRIC,
         "value": mixed_pps,
@@ -152,8 +245,14 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "rr_flux_kernel<mixed>", "kernel_ms": res['mixed']['kern_ms'],
                      "algorithmic_bytes": kbytes,
-                     "fp64_kernel_ms": res['fp64']['kern_ms']},
+                     "fp64_kernel_ms": res['fp64']['kern_ms'],
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE) if traffic is not None else None},
     }
+    if not args.no_serve:
+        srv = serving_leg(torch, dev, args.serve_requests, args.serve_rate)
+        line["serving"] = srv
+        line["served_p50_ms"] = srv["batched"]["p50_ms"]
+        line["served_p99_ms"] = srv["batched"]["p99_ms"]
     if not args.no_cpu:
         cb = cpu_baseline(wl, args.cpu_reps)
         line["cpu_baseline"] = cb

@@ -102,6 +102,16 @@ typedef struct rr_params {
     const double* w;          /* [npv]                                               */
 } rr_params;
 
+/* One model evaluation of a batch: the arguments of rr_evaluate for one request. */
+typedef struct rr_request {
+    rr_lcdata lc;
+    rr_params pv;
+    rr_ld ld;
+    void* workspace;          /* rr_workspace_bytes(...) bytes, device               */
+    size_t workspace_bytes;
+    void* flux;               /* [npv, npt], double (RR_FP64) or float (RR_MIXED)    */
+} rr_request;
+
 const char* rr_last_error(void);
 int rr_abi_version(void);
 
@@ -131,7 +141,8 @@ int rr_integrate_profiles(const double* mu, int nmu, const double* ldp, int npv,
                           double* istar, void* stream);
 
 /* Bytes of device workspace rr_evaluate / rr_setup / rr_flux need for these sizes:
- * per-pv state, limb-darkening means and the queue of in-window points. */
+ * the per-pv state (orbit coefficients, contact window, validity), the stellar
+ * disk integrals and the limb-darkening means. */
 size_t rr_workspace_bytes(int npv, int npt, int npb, int ng, int nlc);
 
 /* The full RoadRunner model: per-pv setup (validity, limb-darkening profiles and
@@ -152,6 +163,16 @@ int rr_setup(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_p
              const rr_ld* ld, void* workspace, size_t workspace_bytes, void* stream);
 int rr_flux(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
             void* workspace, size_t workspace_bytes, void* flux, void* stream);
+
+/* Up to rr_max_batch() independent requests (each its own light curves,
+ * population, limb-darkening law and workspace) evaluated with one setup launch
+ * and at most two flux launches on `stream`; the results equal rr_evaluate's for
+ * each request alone.  Every request is validated before anything is launched.
+ * No counterpart in the reference, which evaluates one model at a time
+ * (rrmodel.py:175-238); this is the launch primitive of the request server. */
+int rr_max_batch(void);
+int rr_evaluate_batch(int precision, const rr_table* tab, int nreq, const rr_request* reqs,
+                      void* stream);
 
 /* Orbit boundary alone: Taylor coefficients coeffs[npv, 2, 5] (row 0 x, row 1 y,
  * coefficient n = d^n/dt^n / n!) and contact window win[npv, 2] = (t1, t4) for

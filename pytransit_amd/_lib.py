@@ -54,6 +54,11 @@ class RRParams(ctypes.Structure):
                 ('e', c_void_p), ('w', c_void_p)]
 
 
+class RRRequest(ctypes.Structure):
+    _fields_ = [('lc', RRLcData), ('pv', RRParams), ('ld', RRLd), ('workspace', c_void_p),
+                ('workspace_bytes', c_size_t), ('flux', c_void_p)]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     'rr_last_error': (ctypes.c_char_p, []),
@@ -76,6 +81,8 @@ _SIGNATURES = {
                           c_void_p, c_void_p, c_void_p, c_void_p]),
     'ts_evaluate': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd),
                             c_void_p, c_size_t, c_void_p, c_void_p]),
+    'rr_max_batch': (c_int, []),
+    'rr_evaluate_batch': (c_int, [c_int, POINTER(RRTable), c_int, POINTER(RRRequest), c_void_p]),
 }
 
 _lib = None

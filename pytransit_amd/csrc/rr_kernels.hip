@@ -280,9 +280,32 @@ __device__ void setup_ld_role(const SetupArgs& s, int pv) {
     }
 }
 
-__global__ void __launch_bounds__(64) rr_setup_kernel(SetupArgs s) {
-    if ((int)blockIdx.x < s.n_orbit_blocks) setup_orbit_role(s, blockIdx.x);
-    else setup_ld_role(s, blockIdx.x - s.n_orbit_blocks);
+__device__ __forceinline__ void setup_body(const SetupArgs& s, int blk) {
+    if (blk < s.n_orbit_blocks) setup_orbit_role(s, blk);
+    else setup_ld_role(s, blk - s.n_orbit_blocks);
+}
+
+__global__ void __launch_bounds__(64) rr_setup_kernel(SetupArgs s) { setup_body(s, blockIdx.x); }
+
+// Several requests in one launch: item i owns blocks [start[i], start[i+1]).  The
+// item index is wave-uniform, so its arguments are scalar loads.
+constexpr int kMaxBatch = 16;
+
+struct SetupBatch {
+    int n;
+    int start[kMaxBatch + 1];
+    SetupArgs a[kMaxBatch];
+};
+
+__device__ __forceinline__ int batch_item(const int* start, int n, int bid) {
+    int i = 0;
+    while (i + 1 < n && bid >= start[i + 1]) ++i;
+    return i;
+}
+
+__global__ void __launch_bounds__(64) rr_setup_batch_kernel(SetupBatch b) {
+    const int i = batch_item(b.start, b.n, blockIdx.x);
+    setup_body(b.a[i], blockIdx.x - b.start[i]);
 }
 
 // ------------------------------------------------------------------------------------
@@ -420,7 +443,7 @@ __device__ __forceinline__ float sample_f32(const float* cf, float kf, float inv
 }
 
 template <bool kMixed, bool kFast>
-__global__ void __launch_bounds__(kThreads) rr_flux_kernel(FluxArgs f) {
+__device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     using Real = typename std::conditional<kMixed, float, double>::type;
     extern __shared__ __align__(16) unsigned char lds[];
     uint16_t* s_q = (uint16_t*)lds;
@@ -428,8 +451,8 @@ __global__ void __launch_bounds__(kThreads) rr_flux_kernel(FluxArgs f) {
     Real* s_dep = (Real*)(s_tc + kMaxChunk);
     Real* s_row = s_dep + (f.uniform_ns > 0 ? f.chunk_pts * f.uniform_ns : 0);
 
-    const int pv = blockIdx.x / f.ntiles;
-    const int tile = blockIdx.x - pv * f.ntiles;
+    const int pv = bid / f.ntiles;
+    const int tile = bid - pv * f.ntiles;
     const int wl = threadIdx.x;
     const int tbase = tile * kTile;
     int its[kRuns], nrun[kRuns];
@@ -590,6 +613,25 @@ __global__ void __launch_bounds__(kThreads) rr_flux_kernel(FluxArgs f) {
     }
 }
 
+template <bool kMixed, bool kFast>
+__global__ void __launch_bounds__(kThreads) rr_flux_kernel(FluxArgs f) {
+    flux_body<kMixed, kFast>(f, blockIdx.x);
+}
+
+struct FluxBatch {
+    int n;
+    int start[kMaxBatch + 1];
+    FluxArgs a[kMaxBatch];
+};
+
+template <bool kMixed, bool kFast>
+__global__ void __launch_bounds__(kThreads) rr_flux_batch_kernel(FluxBatch b) {
+    const int i = batch_item(b.start, b.n, blockIdx.x);
+    flux_body<kMixed, kFast>(b.a[i], blockIdx.x - b.start[i]);
+}
+
+static_assert(sizeof(SetupBatch) <= 4096 && sizeof(FluxBatch) <= 4096, "kernel arguments above 4 KB");
+
 }  // namespace rr
 
 using namespace rr;
@@ -741,27 +783,30 @@ static int validate(const char* who, int precision, const rr_table* tab, const r
     return RR_OK;
 }
 
-int rr_setup(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const rr_ld* ld,
-             void* workspace, size_t workspace_bytes, void* stream) {
-    int rc = validate("rr_setup", precision, tab, lc, pv, workspace_bytes, workspace);
+// Validates one request and fills the setup kernel's arguments; *nblocks = 0 when
+// there is nothing to do.
+static int prepare_setup(const char* who, int precision, const rr_table* tab, const rr_lcdata* lc,
+                         const rr_params* pv, const rr_ld* ld, void* workspace, size_t workspace_bytes,
+                         SetupArgs& sa, int& nblocks) {
+    nblocks = 0;
+    int rc = validate(who, precision, tab, lc, pv, workspace_bytes, workspace);
     if (rc) return rc;
     const int npv = pv->npv, npb = lc->npb;
     if (npv == 0 || lc->npt == 0) return RR_OK;
-    if (!ld) return set_error(RR_EINVAL, "rr_setup: null limb-darkening descriptor");
+    if (!ld) return set_error(RR_EINVAL, "%s: null limb-darkening descriptor", who);
     if (ld->law == RR_LD_TABULATED) {
-        if (!ld->ldp || !ld->istar) return set_error(RR_EINVAL, "rr_setup: tabulated LD needs ldp and istar");
+        if (!ld->ldp || !ld->istar) return set_error(RR_EINVAL, "%s: tabulated LD needs ldp and istar", who);
     } else {
         if (ld->law < RR_LD_UNIFORM || ld->law > RR_LD_POWER_2_PM)
-            return set_error(RR_EINVAL, "rr_setup: unknown limb-darkening law %d", ld->law);
+            return set_error(RR_EINVAL, "%s: unknown limb-darkening law %d", who, ld->law);
         if (ld->nldc < kMinCoeffs[ld->law])
-            return set_error(RR_EINVAL, "rr_setup: law %d needs %d coefficients, got %d", ld->law,
+            return set_error(RR_EINVAL, "%s: law %d needs %d coefficients, got %d", who, ld->law,
                              kMinCoeffs[ld->law], ld->nldc);
-        if (ld->nldc > 0 && !ld->ldc) return set_error(RR_EINVAL, "rr_setup: null ldc");
+        if (ld->nldc > 0 && !ld->ldc) return set_error(RR_EINVAL, "%s: null ldc", who);
     }
     if (!pv->k || !pv->p || !pv->a || !pv->inc || !pv->e || !pv->w)
-        return set_error(RR_EINVAL, "rr_setup: null parameter array");
+        return set_error(RR_EINVAL, "%s: null parameter array", who);
     WsView v = carve(workspace, npv, lc->npt, npb, tab->ng);
-    SetupArgs sa;
     sa.k = pv->k; sa.p = pv->p; sa.a = pv->a; sa.inc = pv->inc; sa.e = pv->e; sa.w = pv->w;
     sa.law = ld->law; sa.nldc = ld->nldc; sa.ldc = ld->ldc; sa.ldp_in = ld->ldp; sa.istar_in = ld->istar;
     sa.W = tab->weights; sa.W32 = tab->weights32; sa.ze = tab->ze; sa.mu = tab->mu;
@@ -771,21 +816,24 @@ int rr_setup(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_p
     sa.npv = npv; sa.npb = npb; sa.kcols = pv->kcols; sa.nk = tab->nk; sa.ng = tab->ng; sa.nz = tab->nz;
     sa.kmin = tab->kmin; sa.kmax = tab->kmax; sa.dk = tab->dk;
     sa.n_orbit_blocks = (npv + kOrbitGroup - 1) / kOrbitGroup;
-    hipLaunchKernelGGL(rr_setup_kernel, dim3(sa.n_orbit_blocks + npv), dim3(64), 0, (hipStream_t)stream, sa);
-    return check_launch("rr_setup_kernel");
+    nblocks = sa.n_orbit_blocks + npv;
+    return RR_OK;
 }
 
-int rr_flux(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, void* workspace,
-            size_t workspace_bytes, void* flux, void* stream) {
-    int rc = validate("rr_flux", precision, tab, lc, pv, workspace_bytes, workspace);
+// Validates one request and fills the flux kernel's arguments, its dynamic LDS
+// bytes and its variant (sorted single light curve or general).
+static int prepare_flux(const char* who, int precision, const rr_table* tab, const rr_lcdata* lc,
+                        const rr_params* pv, void* workspace, size_t workspace_bytes, void* flux, FluxArgs& fa,
+                        long long& nblocks, size_t& lds, bool& fast) {
+    nblocks = 0;
+    int rc = validate(who, precision, tab, lc, pv, workspace_bytes, workspace);
     if (rc) return rc;
     const int npv = pv->npv, npt = lc->npt, npb = lc->npb, nlc = lc->nlc;
     if (npv == 0 || npt == 0) return RR_OK;
     if (!lc->times || !lc->pbids || !lc->epids || !lc->nsamples || !lc->exptimes || (nlc > 1 && !lc->lcids))
-        return set_error(RR_EINVAL, "rr_flux: null light-curve array");
-    if (!pv->k || !pv->t0 || !flux) return set_error(RR_EINVAL, "rr_flux: null parameter array");
+        return set_error(RR_EINVAL, "%s: null light-curve array", who);
+    if (!pv->k || !pv->t0 || !flux) return set_error(RR_EINVAL, "%s: null parameter array", who);
     WsView v = carve(workspace, npv, npt, npb, tab->ng);
-    FluxArgs fa;
     fa.times = lc->times; fa.lcids = lc->lcids; fa.pbids = lc->pbids; fa.epids = lc->epids;
     fa.nsamples = lc->nsamples; fa.exptimes = lc->exptimes;
     fa.k = pv->k; fa.t0 = pv->t0; fa.istar = v.istar;
@@ -798,13 +846,34 @@ int rr_flux(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_pa
     fa.chunk_pts = fa.uniform_ns > 0 ? (kMaxItems / fa.uniform_ns < kMaxChunk ? kMaxItems / fa.uniform_ns : kMaxChunk) : 0;
     fa.row_lds = (nlc == 1 && tab->ng <= 2048) ? 1 : 0;
     fa.ntiles = (npt + kTile - 1) / kTile;
-    const long long nblocks = (long long)fa.ntiles * npv;
-    if (nblocks > 0x7fffffffLL) return set_error(RR_ERANGE, "rr_flux: grid too large");
-    const size_t lds = flux_lds_bytes(mixed, tab->ng, fa.chunk_pts, fa.uniform_ns, fa.row_lds != 0);
+    nblocks = (long long)fa.ntiles * npv;
+    if (nblocks > 0x7fffffffLL) return set_error(RR_ERANGE, "%s: grid too large", who);
+    lds = flux_lds_bytes(mixed, tab->ng, fa.chunk_pts, fa.uniform_ns, fa.row_lds != 0);
+    fast = (nlc == 1) && lc->sorted;
+    return RR_OK;
+}
+
+int rr_setup(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const rr_ld* ld,
+             void* workspace, size_t workspace_bytes, void* stream) {
+    SetupArgs sa;
+    int nblocks = 0;
+    int rc = prepare_setup("rr_setup", precision, tab, lc, pv, ld, workspace, workspace_bytes, sa, nblocks);
+    if (rc || nblocks == 0) return rc;
+    hipLaunchKernelGGL(rr_setup_kernel, dim3(nblocks), dim3(64), 0, (hipStream_t)stream, sa);
+    return check_launch("rr_setup_kernel");
+}
+
+int rr_flux(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, void* workspace,
+            size_t workspace_bytes, void* flux, void* stream) {
+    FluxArgs fa;
+    long long nblocks = 0;
+    size_t lds = 0;
+    bool fast = false;
+    int rc = prepare_flux("rr_flux", precision, tab, lc, pv, workspace, workspace_bytes, flux, fa, nblocks, lds, fast);
+    if (rc || nblocks == 0) return rc;
     hipStream_t s = (hipStream_t)stream;
-    const bool fast = (nlc == 1) && lc->sorted;
     const dim3 grid((unsigned)nblocks), block(kThreads);
-    if (mixed) {
+    if (precision == RR_MIXED) {
         if (fast) hipLaunchKernelGGL((rr_flux_kernel<true, true>), grid, block, lds, s, fa);
         else hipLaunchKernelGGL((rr_flux_kernel<true, false>), grid, block, lds, s, fa);
     } else {
@@ -819,6 +888,70 @@ int rr_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
     int rc = rr_setup(precision, tab, lc, pv, ld, workspace, workspace_bytes, stream);
     if (rc) return rc;
     return rr_flux(precision, tab, lc, pv, workspace, workspace_bytes, flux, stream);
+}
+
+int rr_max_batch(void) { return kMaxBatch; }
+
+int rr_evaluate_batch(int precision, const rr_table* tab, int nreq, const rr_request* reqs, void* stream) {
+    if (nreq < 0 || nreq > kMaxBatch)
+        return set_error(RR_EINVAL, "rr_evaluate_batch: %d requests, at most %d per call", nreq, kMaxBatch);
+    if (nreq > 0 && !reqs) return set_error(RR_EINVAL, "rr_evaluate_batch: null request array");
+    SetupBatch sb;
+    FluxBatch fb[2];  // [0] general, [1] sorted single light curve
+    size_t lds[2] = {0, 0};
+    long long fblocks[2] = {0, 0};
+    long long sblocks = 0;
+    sb.n = 0;
+    fb[0].n = fb[1].n = 0;
+    char who[48];
+    // validate every request before launching anything
+    for (int r = 0; r < nreq; ++r) {
+        const rr_request& q = reqs[r];
+        snprintf(who, sizeof(who), "rr_evaluate_batch[%d]", r);
+        SetupArgs sa;
+        int nb = 0;
+        int rc = prepare_setup(who, precision, tab, &q.lc, &q.pv, &q.ld, q.workspace, q.workspace_bytes, sa, nb);
+        if (rc) return rc;
+        FluxArgs fa;
+        long long nf = 0;
+        size_t l = 0;
+        bool fast = false;
+        rc = prepare_flux(who, precision, tab, &q.lc, &q.pv, q.workspace, q.workspace_bytes, q.flux, fa, nf, l, fast);
+        if (rc) return rc;
+        if (nb == 0) continue;
+        sb.start[sb.n] = (int)sblocks;
+        sb.a[sb.n++] = sa;
+        sblocks += nb;
+        FluxBatch& b = fb[fast ? 1 : 0];
+        b.start[b.n] = (int)fblocks[fast ? 1 : 0];
+        b.a[b.n++] = fa;
+        fblocks[fast ? 1 : 0] += nf;
+        lds[fast ? 1 : 0] = l > lds[fast ? 1 : 0] ? l : lds[fast ? 1 : 0];
+    }
+    if (sblocks > 0x7fffffffLL || fblocks[0] > 0x7fffffffLL || fblocks[1] > 0x7fffffffLL)
+        return set_error(RR_ERANGE, "rr_evaluate_batch: grid too large");
+    if (sb.n == 0) return RR_OK;
+    hipStream_t s = (hipStream_t)stream;
+    sb.start[sb.n] = (int)sblocks;
+    hipLaunchKernelGGL(rr_setup_batch_kernel, dim3((unsigned)sblocks), dim3(64), 0, s, sb);
+    int rc = check_launch("rr_setup_batch_kernel");
+    if (rc) return rc;
+    const bool mixed = precision == RR_MIXED;
+    for (int v = 0; v < 2; ++v) {
+        if (fb[v].n == 0) continue;
+        fb[v].start[fb[v].n] = (int)fblocks[v];
+        const dim3 grid((unsigned)fblocks[v]), block(kThreads);
+        if (mixed) {
+            if (v) hipLaunchKernelGGL((rr_flux_batch_kernel<true, true>), grid, block, lds[v], s, fb[v]);
+            else hipLaunchKernelGGL((rr_flux_batch_kernel<true, false>), grid, block, lds[v], s, fb[v]);
+        } else {
+            if (v) hipLaunchKernelGGL((rr_flux_batch_kernel<false, true>), grid, block, lds[v], s, fb[v]);
+            else hipLaunchKernelGGL((rr_flux_batch_kernel<false, false>), grid, block, lds[v], s, fb[v]);
+        }
+        rc = check_launch("rr_flux_batch_kernel");
+        if (rc) return rc;
+    }
+    return RR_OK;
 }
 
 }  // extern "C"

@@ -13,6 +13,10 @@ density through the scaled semi-major axis relation (pytransit/orbits/orbits_py.
                        profiles (TSModel).
 * ``rr_gradient``   -- 2,048 pv x 10,000 times, quadratic LD, 5 supersamples, with a
                        synthetic observed light curve.
+* ``serving_stream`` -- Poisson arrivals of fit requests: light-curve sets of
+                       log-uniform 1k-200k points in 1-64 passbands, populations of
+                       log-uniform 32-8,192 pv, quadratic / power-2 / nonlinear-4
+                       LD, nss in {1, 5, 10}.
 """
 import math
 from types import SimpleNamespace
@@ -121,3 +125,58 @@ def rr_gradient(npv=2048, npt=10000, seed=3, exptime=0.01, noise=300e-6):
     return SimpleNamespace(name='rr_gradient', law='quadratic', times=np.linspace(-0.35, 0.35, npt), nsamples=5,
                            exptimes=exptime, truth=truth, pop=pop, noise=noise,
                            noise_draw=rng.normal(0.0, noise, npt))
+
+
+LAW_DRAWS = ('quadratic', 'power-2', 'nonlinear')
+
+
+def _ld_draw(rng, law, npv, npb):
+    """Limb-darkening coefficients [npv, npb, nldc] for one request."""
+    if law == 'quadratic':
+        return np.stack([rng.uniform(0.1, 0.6, (npv, npb)), rng.uniform(0.0, 0.3, (npv, npb))], -1)
+    if law == 'power-2':
+        return np.stack([rng.uniform(0.3, 0.9, (npv, npb)), rng.uniform(0.3, 1.0, (npv, npb))], -1)
+    # four-parameter nonlinear law near solar-type values
+    base = np.array([0.56, -0.08, 0.67, -0.33])
+    return base + rng.normal(0.0, 0.02, (npv, npb, 4))
+
+
+def serving_dataset(rng, npt_range=(1000, 200000), npb_max=64):
+    """One light-curve set: npt ~ log-uniform, 2-min cadence from BJD 2459000.0,
+    split into npb contiguous light curves, one per passband; nss in {1, 5, 10}
+    with a 30-min exposure when nss > 1."""
+    npt = int(round(math.exp(rng.uniform(math.log(npt_range[0]), math.log(npt_range[1])))))
+    npb = int(rng.integers(1, npb_max + 1))
+    npb = min(npb, npt)
+    nss = int(rng.choice([1, 5, 10]))
+    times = 2459000.0 + np.arange(npt) * (2.0 / 1440.0)
+    lcids = (np.arange(npt) * npb) // npt
+    return dict(time=times, lcids=lcids, pbids=np.arange(npb), nsamples=nss,
+                exptimes=(30.0 / 1440.0) if nss > 1 else 0.0)
+
+
+def serving_request(rng, npb, npv_range=(32, 8192)):
+    """One fit request's population on a dataset with npb passbands."""
+    npv = int(round(math.exp(rng.uniform(math.log(npv_range[0]), math.log(npv_range[1])))))
+    law = str(rng.choice(LAW_DRAWS))
+    k, p, a, inc, t0 = _orbit_draws(rng, npv, 13.0)
+    kpb = k[:, None] + 0.002 * np.sin(np.arange(npb) / 8.0)[None, :]
+    return dict(k=kpb, ldc=_ld_draw(rng, law, npv, npb), t0=2459000.0 + t0, p=p, a=a, i=inc, ld=law)
+
+
+def serving_stream(n_requests=300, rate=2000.0, n_datasets=24, seed=4, npt_range=(1000, 200000),
+                   npv_range=(32, 8192), npb_max=64):
+    """A seeded request stream: ``datasets`` (set_data keyword dicts) and
+    ``requests`` (submit keyword dicts with ``dataset`` index and ``arrival`` in s
+    from the start, Poisson at ``rate`` per second).  Each request draws its
+    light-curve set uniformly from the pool, so sets are reused and stay resident."""
+    rng = np.random.default_rng(seed)
+    datasets = [serving_dataset(rng, npt_range, npb_max) for _ in range(n_datasets)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    requests = []
+    for j in range(n_requests):
+        d = int(rng.integers(n_datasets))
+        r = serving_request(rng, datasets[d]['pbids'].size, npv_range)
+        r.update(dataset=d, arrival=float(arrivals[j]))
+        requests.append(r)
+    return SimpleNamespace(name='serving_stream', datasets=datasets, requests=requests, rate=rate)

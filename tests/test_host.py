@@ -55,6 +55,15 @@ def test_c_abi_rejects_bad_arguments():
     assert rc == -1 and b'coefficients' in lib.rr_last_error()
     assert lib.rr_workspace_bytes(4, 1000, 1, 100, 1) >= 4 * 128 + 4 * 100 * 12
     assert lib.ts_workspace_bytes(4, 8, 100, 40, 1) >= 4 * 8 * 100 * 4 + 4 * 40 * 112 * 4
+    # batches: size limit and an empty batch are checked before any device work
+    assert lib.rr_max_batch() == 16
+    rc = lib.rr_evaluate_batch(1, None, 17, None, None)
+    assert rc == -1 and b'at most 16' in lib.rr_last_error()
+    assert lib.rr_evaluate_batch(1, None, 0, None, None) == 0
+    reqs = (_lib.RRRequest * 2)()
+    reqs[1].pv.npv = -1
+    rc = lib.rr_evaluate_batch(1, None, 2, reqs, None)
+    assert rc == -1 and b'rr_evaluate_batch[0]' in lib.rr_last_error()
 
 
 # ---- set_data (transitmodel.py:56-125) ----------------------------------------------

@@ -1,0 +1,98 @@
+"""The request server on the GPU: every request's flux is the same in a mixed,
+continuously batched stream as when evaluated alone by RoadRunnerModel, and the
+batched kernels agree with the FP64 oracle.
+
+Tolerance of alone-vs-mix: 0 (bit-identical).  A batch runs the same per-request
+kernels with the same arithmetic; only the launch that carries them differs.
+"""
+import numpy as np
+import pytest
+import torch
+
+from pytransit_amd import RoadRunnerModel, workloads
+from pytransit_amd.serving import Dataset, RRServer
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def small_stream(seed=11, n=24):
+    return workloads.serving_stream(n_requests=n, rate=1e4, n_datasets=5, seed=seed, npt_range=(500, 20000),
+                                    npv_range=(8, 300), npb_max=4)
+
+
+def alone(ds_kw, rq, prec):
+    tm = RoadRunnerModel(rq['ld'], precision=prec)
+    tm.set_data(**ds_kw)
+    f = tm._evaluate_device(rq['k'], rq['ldc'], rq['t0'], rq['p'], rq['a'], rq['i'], 0.0, 0.0)
+    return f.cpu().numpy()
+
+
+def submit(srv, dsets, rq):
+    return srv.submit(dsets[rq['dataset']], rq['k'], rq['ldc'], rq['t0'], rq['p'], rq['a'], rq['i'], ld=rq['ld'])
+
+
+@pytest.mark.parametrize('prec', ['mixed', 'fp64'])
+def test_mix_equals_alone(gpu, prec):
+    st = small_stream()
+    say(f'case: serving, {len(st.requests)} requests, {prec}, all submitted at once')
+    dsets = [Dataset(**d) for d in st.datasets]
+    srv = RRServer(hbm_budget=1 << 30, slots=4, max_batch=16, precision=prec)
+    reqs = [submit(srv, dsets, rq) for rq in st.requests]
+    srv.run(timeout=60)
+    assert max(r.batch_size for r in reqs) > 1, 'requests were batched'
+    for r, rq in zip(reqs, st.requests):
+        ref = alone(st.datasets[rq['dataset']], rq, prec)
+        got = r.result()
+        assert got.shape == ref.shape
+        np.testing.assert_array_equal(got, ref, err_msg=f'request {r.rid} ({rq["ld"]}, npv={r.npv}, '
+                                                        f'npt={r.dataset.npt}, npb={r.dataset.npb})')
+
+
+def test_continuous_arrivals_tight_budget(gpu):
+    st = small_stream(seed=12, n=30)
+    say(f'case: serving, {len(st.requests)} requests in waves, budget for a few at a time')
+    dsets = [Dataset(**d) for d in st.datasets]
+    need = max(rq['k'].shape[0] * st.datasets[rq['dataset']]['time'].size for rq in st.requests) * 4
+    budget = 3 * need + sum(d.nbytes for d in dsets) + (32 << 20)
+    srv = RRServer(hbm_budget=budget, slots=3, max_batch=5, precision='mixed')
+    reqs = []
+    for wave in (st.requests[:10], st.requests[10:20], st.requests[20:]):
+        reqs += [submit(srv, dsets, rq) for rq in wave]
+        for _ in range(3):
+            srv.step()
+        assert srv._arena.in_use <= srv._arena.capacity
+    srv.run(timeout=60)
+    assert all(r.done for r in reqs)
+    for r, rq in zip(reqs, st.requests):
+        np.testing.assert_array_equal(r.result(), alone(st.datasets[rq['dataset']], rq, 'mixed'))
+        r.release()
+    assert srv._arena.in_use == sum(d.nbytes for d in dsets if d.offset is not None)
+
+
+def test_batched_kernels_against_oracle(gpu, oracle):
+    say('case: serving batch of 3 small requests vs the FP64 oracle')
+    rng = np.random.default_rng(5)
+    ds_kw = [workloads.serving_dataset(rng, (400, 900), 3) for _ in range(3)]
+    rqs = []
+    for d, law in zip(ds_kw, workloads.LAW_DRAWS):
+        rq = workloads.serving_request(rng, d['pbids'].size, (3, 6))
+        rq.update(ld=law, ldc=workloads._ld_draw(rng, law, rq['p'].size, d['pbids'].size))
+        rqs.append(rq)
+    for prec, tol in (('fp64', 1e-9), ('mixed', 1e-6)):
+        srv = RRServer(hbm_budget=256 << 20, slots=1, max_batch=16, precision=prec)
+        reqs = [srv.submit(Dataset(**d), rq['k'], rq['ldc'], rq['t0'], rq['p'], rq['a'], rq['i'], ld=rq['ld'])
+                for d, rq in zip(ds_kw, rqs)]
+        srv.run(timeout=60)
+        assert reqs[0].batch_size == 3
+        for r, d, rq in zip(reqs, ds_kw, rqs):
+            om = oracle.OracleRR(rq['ld'])
+            om.set_data(**d)
+            ref = om.evaluate(rq['k'], rq['ldc'], rq['t0'][:, None], rq['p'], rq['a'], rq['i'])
+            got = r.result().astype(np.float64)
+            err = np.abs(got - ref).max()
+            say(f'  {prec} {rq["ld"]:10s} npv={r.npv} npt={r.dataset.npt} npb={r.dataset.npb} max err {err:.2e}')
+            assert err <= tol
