@@ -14,7 +14,15 @@ reported in ppm.  ``roofline`` is for the flux kernel alone, timed with HIP even
 on the stream it is launched on; ``cpu_baseline`` is the C/OpenMP FP64
 restatement of the reference algorithm on this host's cores.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+Further legs, reported beside the headline in the same JSON line:
+* ``ts_many_pb``: TSModel, 1,024 pv x 5,000 times x 512 passbands (mixed and FP64);
+* ``gradient``: fused log-likelihood with its adjoint gradient, 2,048 pv x 10,000
+  times (mixed and FP64);
+* ``serving``: a Poisson stream of 300 fit requests through the request server,
+  batched (4 slots, up to 16 requests per launch) and unbatched (one at a time),
+  with throughput and p50/p99 latency.
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--no-serve] [--no-extra]
 """
 import argparse
 import json
@@ -38,6 +46,8 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--cpu-reps', type=int, default=5)
     ap.add_argument('--no-serve', action='store_true', help='skip the serving leg')
+    ap.add_argument('--no-extra', action='store_true',
+                    help='skip the many-passband TSModel and likelihood-gradient legs')
     ap.add_argument('--serve-requests', type=int, default=300)
     ap.add_argument('--serve-rate', type=float, default=2000.0, help='Poisson arrival rate, requests/s')
     ap.add_argument('--copy-probe', action='store_true',
@@ -148,6 +158,81 @@ def serving_leg(torch, dev, nreq, rate):
     return out
 
 
+def _timed(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / steps, out
+
+
+def ts_leg(torch, dev, steps, warmup):
+    """Many-passband TSModel: 1,024 pv x 5,000 times x 512 passbands, tabulated
+    128-node profiles, exact weights at mean(k) per pv."""
+    from pytransit_amd import TabulatedLDModel, TSModel, workloads
+    probe = TSModel('quadratic', nzin=64, nzlimb=64)
+    wl = workloads.ts_many_pb(mu=probe.mu)
+    ldm = TabulatedLDModel(torch.as_tensor(wl.ldp, device=dev), torch.as_tensor(wl.istar, device=dev))
+    args = tuple(torch.as_tensor(x, device=dev) for x in (wl.k, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w))
+    res = {}
+    for prec in ('mixed', 'fp64'):
+        tm = TSModel(ldm, nzin=64, nzlimb=64, precision=prec, device=dev)
+        tm.set_data(wl.times)
+        k, t0, p, a, i, e, w = args
+        sec, out = _timed(torch, lambda: tm._evaluate_device(k, None, t0, p, a, i, e, w), steps, warmup)
+        res[prec] = (sec, out)
+    npv, npb, npt = res['mixed'][1].shape
+    err = 0.0
+    for j in range(0, npv, 64):  # chunked to bound memory
+        d = (res['mixed'][1][j:j + 64].double() - res['fp64'][1][j:j + 64]).abs().max()
+        err = max(err, float(d))
+    pts = npv * npb * npt
+    wbytes = pts * 4
+    return {"workload": f"TSModel, npv={npv} x npb={npb} x npt={npt}, 128 mu nodes, tabulated LD "
+                        f"(LDModel), exact weights at mean(k)",
+            "points": pts, "mixed_points_per_s": pts / res['mixed'][0], "fp64_points_per_s": pts / res['fp64'][0],
+            "mixed_ms": res['mixed'][0] * 1e3, "fp64_ms": res['fp64'][0] * 1e3,
+            "mixed_over_fp64": res['fp64'][0] / res['mixed'][0], "max_abs_err_ppm_mixed_vs_fp64": err * 1e6,
+            "flux_write_floor_ms": wbytes / (HBM_PEAK_GBS * 1e9) * 1e3}
+
+
+def grad_leg(torch, dev, steps, warmup):
+    """Log-likelihood and its adjoint gradient: 2,048 pv x 10,000 times, nss=5,
+    quadratic LD, 300 ppm noise."""
+    from pytransit_amd import RoadRunnerModel, workloads
+    wl = workloads.rr_gradient()
+    tr = wl.truth
+    names = ('k', 't0', 'p', 'a', 'i', 'u1', 'u2')
+    theta = torch.as_tensor(np.stack([wl.pop[n] for n in names], 1), device=dev)
+    res = {}
+    for prec in ('mixed', 'fp64'):
+        tm = RoadRunnerModel('quadratic', precision=prec, device=dev)
+        tm.set_data(wl.times, nsamples=wl.nsamples, exptimes=wl.exptimes)
+        if prec == 'mixed':
+            ref = RoadRunnerModel('quadratic', precision='fp64', device=dev)
+            ref.set_data(wl.times, nsamples=wl.nsamples, exptimes=wl.exptimes)
+            f_true = ref._evaluate_device([[tr['k']]], [[[tr['u1'], tr['u2']]]], [[tr['t0']]], tr['p'], tr['a'],
+                                          tr['i'], 0.0, 0.0)[0]
+            obs = f_true + torch.as_tensor(wl.noise_draw, device=dev)
+        th = theta
+        fn = lambda: tm._lnlike_device(th[:, 0], th[:, 5:7].reshape(-1, 1, 2), th[:, 1], th[:, 2], th[:, 3],
+                                       th[:, 4], obs, wl.noise, grad=True)
+        res[prec] = _timed(torch, fn, steps, warmup)
+    npv = theta.shape[0]
+    gm, gf = res['mixed'][1][1], res['fp64'][1][1]
+    scale = gf.abs().pow(2).mean(0).sqrt()
+    rel = ((gm - gf).abs() / (gf.abs() + scale)).max()
+    return {"workload": f"lnL + d lnL/d(k,t0,p,a,i,u1,u2), npv={npv} x npt={wl.times.size}, nss=5, quadratic",
+            "mixed_ms": res['mixed'][0] * 1e3, "fp64_ms": res['fp64'][0] * 1e3,
+            "mixed_points_per_s": npv * wl.times.size / res['mixed'][0],
+            "fp64_points_per_s": npv * wl.times.size / res['fp64'][0],
+            "max_rel_grad_diff_mixed_vs_fp64": float(rel),
+            "max_abs_lnl_diff_mixed_vs_fp64": float((res['mixed'][1][0] - res['fp64'][1][0]).abs().max())}
+
+
 def workloads_mod():
     from pytransit_amd import workloads
     return workloads
@@ -248,6 +333,9 @@ def main():
                      "fp64_kernel_ms": res['fp64']['kern_ms'],
                      "traffic_source": os.path.relpath(TRAFFIC_FILE) if traffic is not None else None},
     }
+    if not args.no_extra:
+        line["ts_many_pb"] = ts_leg(torch, dev, max(3, args.steps // 4), 1)
+        line["gradient"] = grad_leg(torch, dev, max(3, args.steps // 4), 1)
     if not args.no_serve:
         srv = serving_leg(torch, dev, args.serve_requests, args.serve_rate)
         line["serving"] = srv

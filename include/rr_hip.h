@@ -77,7 +77,8 @@ typedef struct rr_ld {
 /* Light-curve data set by set_data() (transitmodel.py:56-125), device-resident. */
 typedef struct rr_lcdata {
     int32_t npt, nlc, npb, nep;
-    int32_t sorted;           /* 1 if times are non-decreasing (enables run skipping) */
+    int32_t sorted;           /* 1 if lcids are non-decreasing and the times of each light
+                                 curve are non-decreasing (enables run skipping)          */
     int32_t uniform_nsamples; /* the common nsamples of all light curves, or 0         */
     const double* times;      /* [npt]                                               */
     const int32_t* lcids;     /* [npt], values in [0, nlc)                           */

@@ -10,13 +10,13 @@
 //
 // Setup is one launch with two block roles: orbit blocks (8 pv per 64-lane
 // block, the 7 Kepler solves of a pv on 7 lanes) and limb-darkening blocks (one
-// pv per block: profiles at the mu nodes in LDS, disk integral, and the
-// table-lerp contraction with lanes over g reading the table coalesced).
+// (pv, passband) pair per block: profile at the mu nodes in LDS, disk integral,
+// and the table-lerp contraction with lanes over g reading the table coalesced).
 //
 // The flux kernel runs one wavefront per workgroup over a tile of 1024 times of
 // one pv; each lane owns two runs of 8 consecutive times and issues all its time
-// loads up front.  With sorted times and one light curve, a lane classifies each
-// run from the phases of its first and last time: runs entirely outside the
+// loads up front.  With contiguous light curves sorted in time, a lane classifies
+// each run from the phases of its first and last time: runs entirely outside the
 // transit window store 1.0 as vectors.  In-window points are compacted into a
 // per-wave LDS queue (ballot + popcount, count kept in a register), and all 64
 // lanes then share the supersampled geometry of the queued points, reading the
@@ -198,7 +198,9 @@ __device__ void setup_orbit_role(const SetupArgs& s, int blk) {
     }
 }
 
-__device__ void setup_ld_role(const SetupArgs& s, int pv) {
+// One (pv, passband) pair per block: the profile at the mu nodes, its disk
+// integral and the limb-darkening means of that passband.
+__device__ void setup_ld_role(const SetupArgs& s, int pv, int ipb) {
     __shared__ double s_l[kMaxNz];
     const int lane = threadIdx.x;
     const int nz = s.nz, ng = s.ng;
@@ -212,77 +214,79 @@ __device__ void setup_ld_role(const SetupArgs& s, int pv) {
         ak = (k0 - s.kmin - ik * s.dk) / s.dk;
     }
     const bool mixed = s.ldm32 != nullptr;
-    for (int ipb = 0; ipb < s.npb; ++ipb) {
-        const size_t j = (size_t)pv * s.npb + ipb;
-        // profile at the mu nodes and its disk integral
-        double ist;
-        if (s.law == RR_LD_TABULATED) {
-            for (int iz = lane; iz < nz; iz += 64) s_l[iz] = s.ldp_in[j * nz + iz];
-            ist = s.istar_in[j];
-        } else {
-            const double* c = s.ldc + j * s.nldc;
-            for (int iz = lane; iz < nz; iz += 64) s_l[iz] = ld_law(s.law, s.mu[iz], c, s.nldc);
-            if (!ld_integral(s.law, c, ist)) {
-                double part = 0.0;
-                for (int m = 1 + lane; m < 200; m += 64) part += ld_trapz_segment(s.law, c, s.nldc, m);
-                ist = 2.0 * kPi * wave_sum(part);
-            }
+    const size_t j = (size_t)pv * s.npb + ipb;
+    // profile at the mu nodes and its disk integral
+    double ist;
+    if (s.law == RR_LD_TABULATED) {
+        for (int iz = lane; iz < nz; iz += 64) s_l[iz] = s.ldp_in[j * nz + iz];
+        ist = s.istar_in[j];
+    } else {
+        const double* c = s.ldc + j * s.nldc;
+        for (int iz = lane; iz < nz; iz += 64) s_l[iz] = ld_law(s.law, s.mu[iz], c, s.nldc);
+        if (!ld_integral(s.law, c, ist)) {
+            double part = 0.0;
+            for (int m = 1 + lane; m < 200; m += 64) part += ld_trapz_segment(s.law, c, s.nldc, m);
+            ist = 2.0 * kPi * wave_sum(part);
         }
-        __syncthreads();
-        if (lane == 0) {
-            s.istar[j] = ist;
-            if (ipb == 0) s.rec[pv].ldok = isnan(s_l[0]) ? 0 : 1;
-        }
-        // limb-darkening means: lerp in k of two table rows, or exact 2-D weights
-        // when k0 is outside [kmin, kmax]; model_full.py:47-55
-        for (int ig = lane; ig < ng; ig += 64) {
-            double v;
-            if (in_table) {
-                double d0 = 0.0, d1 = 0.0;
-                if (mixed && s.W32) {
-                    const float* w0 = s.W32 + (size_t)ik * nz * ng + ig;
-                    const float* w1 = w0 + (size_t)nz * ng;
-                    for (int iz = 0; iz < nz; ++iz) {
-                        d0 += (double)w0[(size_t)iz * ng] * s_l[iz];
-                        d1 += (double)w1[(size_t)iz * ng] * s_l[iz];
-                    }
-                } else {
-                    const double* w0 = s.W + (size_t)ik * nz * ng + ig;
-                    const double* w1 = w0 + (size_t)nz * ng;
-                    for (int iz = 0; iz < nz; ++iz) {
-                        d0 += w0[(size_t)iz * ng] * s_l[iz];
-                        d1 += w1[(size_t)iz * ng] * s_l[iz];
-                    }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        s.istar[j] = ist;
+        if (ipb == 0) s.rec[pv].ldok = isnan(s_l[0]) ? 0 : 1;
+    }
+    // limb-darkening means: lerp in k of two table rows, or exact 2-D weights
+    // when k0 is outside [kmin, kmax]; model_full.py:47-55
+    for (int ig = lane; ig < ng; ig += 64) {
+        double v;
+        if (in_table) {
+            double d0 = 0.0, d1 = 0.0;
+            if (mixed && s.W32) {
+                const float* w0 = s.W32 + (size_t)ik * nz * ng + ig;
+                const float* w1 = w0 + (size_t)nz * ng;
+                for (int iz = 0; iz < nz; ++iz) {
+                    d0 += (double)w0[(size_t)iz * ng] * s_l[iz];
+                    d1 += (double)w1[(size_t)iz * ng] * s_l[iz];
                 }
-                v = (1.0 - ak) * d0 + ak * d1;
             } else {
-                // common.py:152-185: normalised annulus areas, recomputed in a second pass
-                const double g = linspace_at(0.0, 1.0 - 1e-7, ng, ig);
-                const double b = g * (1.0 + k0);
-                double a0 = cci_area(s.ze[0], k0, b), sum = a0;
-                for (int iz = 1; iz < nz; ++iz) {
-                    const double a1 = cci_area(s.ze[iz], k0, b);
-                    sum += a1 - a0;
-                    a0 = a1;
-                }
-                double b0 = cci_area(s.ze[0], k0, b);
-                v = (b0 / sum) * s_l[0];
-                for (int iz = 1; iz < nz; ++iz) {
-                    const double b1 = cci_area(s.ze[iz], k0, b);
-                    v += ((b1 - b0) / sum) * s_l[iz];
-                    b0 = b1;
+                const double* w0 = s.W + (size_t)ik * nz * ng + ig;
+                const double* w1 = w0 + (size_t)nz * ng;
+                for (int iz = 0; iz < nz; ++iz) {
+                    d0 += w0[(size_t)iz * ng] * s_l[iz];
+                    d1 += w1[(size_t)iz * ng] * s_l[iz];
                 }
             }
-            if (mixed) s.ldm32[j * ng + ig] = (float)v;
-            else s.ldm64[j * ng + ig] = v;
+            v = (1.0 - ak) * d0 + ak * d1;
+        } else {
+            // common.py:152-185: normalised annulus areas, recomputed in a second pass
+            const double g = linspace_at(0.0, 1.0 - 1e-7, ng, ig);
+            const double b = g * (1.0 + k0);
+            double a0 = cci_area(s.ze[0], k0, b), sum = a0;
+            for (int iz = 1; iz < nz; ++iz) {
+                const double a1 = cci_area(s.ze[iz], k0, b);
+                sum += a1 - a0;
+                a0 = a1;
+            }
+            double b0 = cci_area(s.ze[0], k0, b);
+            v = (b0 / sum) * s_l[0];
+            for (int iz = 1; iz < nz; ++iz) {
+                const double b1 = cci_area(s.ze[iz], k0, b);
+                v += ((b1 - b0) / sum) * s_l[iz];
+                b0 = b1;
+            }
         }
-        __syncthreads();
+        if (mixed) s.ldm32[j * ng + ig] = (float)v;
+        else s.ldm64[j * ng + ig] = v;
     }
 }
 
 __device__ __forceinline__ void setup_body(const SetupArgs& s, int blk) {
-    if (blk < s.n_orbit_blocks) setup_orbit_role(s, blk);
-    else setup_ld_role(s, blk - s.n_orbit_blocks);
+    if (blk < s.n_orbit_blocks) {
+        setup_orbit_role(s, blk);
+    } else {
+        const int b = blk - s.n_orbit_blocks;
+        const int pv = b / s.npb;
+        setup_ld_role(s, pv, b - pv * s.npb);
+    }
 }
 
 __global__ void __launch_bounds__(64) rr_setup_kernel(SetupArgs s) { setup_body(s, blockIdx.x); }
@@ -316,13 +320,13 @@ __global__ void __launch_bounds__(64) rr_setup_batch_kernel(SetupBatch b) {
 // times) and issues its time loads up front.
 //
 // Phase A: the FP64 epoch reduction classifies points against the transit window.
-// With sorted times and one light curve a lane classifies a whole run from the
+// With contiguous light curves sorted in time a lane classifies a whole run from the
 // phases of its first and last time; when the whole wave's run step is outside
 // the window (the common case) the wave stores 1.0 cooperatively, each store
 // instruction covering 1 KB of contiguous memory.  In-window points go to an LDS
 // queue (ballot + popcount; the count is wave-uniform and kept in a register).
 //
-// Phase B: with one nsamples for all light curves, work items are (point, sample)
+// Phase B: with one light curve and one nsamples, work items are (point, sample)
 // pairs in sample order, so a round of 64 lanes covers ~64/nss neighbouring
 // points and the branch mix (outside / inside / limb) stays coherent; per-point
 // sums are formed in sample order from LDS, so results do not depend on the
@@ -484,16 +488,28 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
 
     // ---- phase A -----------------------------------------------------------------
     if (kFast) {
-        const PointCtx c = point_ctx(f, r, pv, 0);
+        // light curves are contiguous and each is sorted in time: a run whose first
+        // and last points share a light curve lies in that light curve, in order
+        const PointCtx c1 = point_ctx(f, r, pv, 0);
 #pragma unroll
         for (int m = 0; m < kRuns; ++m) {
-            bool all_out = true;
+            bool all_out = true, one_lc = true;
+            PointCtx c = c1;
+            if (nrun[m] > 0 && f.nlc > 1) {
+                const int la = min(max(f.lcids[its[m]], 0), f.nlc - 1);
+                const int lb = min(max(f.lcids[its[m] + nrun[m] - 1], 0), f.nlc - 1);
+                one_lc = la == lb;
+                c = point_ctx(f, r, pv, la);
+            }
             if (nrun[m] > 0) {
-                double ef, el;
-                const double tcf = phase_of(tf[m], c.t0v, p, inv_p, ef);
-                const double tcl = phase_of(tl[m], c.t0v, p, inv_p, el);
-                all_out = (ef == el) ? (tcl < c.lo || tcf > c.hi)
-                                     : ((el == ef + 1.0) ? (tcf > c.hi && tcl < c.lo) : false);
+                all_out = false;
+                if (one_lc) {
+                    double ef, el;
+                    const double tcf = phase_of(tf[m], c.t0v, p, inv_p, ef);
+                    const double tcl = phase_of(tl[m], c.t0v, p, inv_p, el);
+                    all_out = (ef == el) ? (tcl < c.lo || tcf > c.hi)
+                                         : ((el == ef + 1.0) ? (tcf > c.hi && tcl < c.lo) : false);
+                }
             }
             const int step0 = tbase + m * kThreads * kRun;
             if (__ballot(!all_out) == 0ull) {  // the wave's whole run step is outside the window
@@ -506,9 +522,10 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                 bool inw = false;
                 const int it = its[m] + j;
                 if (!all_out && j < nrun[m]) {
+                    const PointCtx cc = one_lc ? c : point_ctx(f, r, pv, min(max(f.lcids[it], 0), f.nlc - 1));
                     double ep;
-                    const double tc = phase_of(f.times[it], c.t0v, p, inv_p, ep);
-                    inw = (c.lo <= tc) && (tc <= c.hi);
+                    const double tc = phase_of(f.times[it], cc.t0v, p, inv_p, ep);
+                    inw = (cc.lo <= tc) && (tc <= cc.hi);
                     if (!inw) out[it] = Real(1);
                 }
                 push(inw, it);
@@ -543,7 +560,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     }
     __syncthreads();  // queue entries and the staged row are read by other lanes
     const float inv_dg = (float)(1.0 / f.dg);
-    if (f.uniform_ns > 0 && (kFast || f.nlc == 1)) {
+    if (f.uniform_ns > 0 && f.nlc == 1) {
         // one light curve: the per-point context is uniform
         const int ns = f.uniform_ns;
         const PointCtx c = point_ctx(f, r, pv, 0);
@@ -816,12 +833,14 @@ static int prepare_setup(const char* who, int precision, const rr_table* tab, co
     sa.npv = npv; sa.npb = npb; sa.kcols = pv->kcols; sa.nk = tab->nk; sa.ng = tab->ng; sa.nz = tab->nz;
     sa.kmin = tab->kmin; sa.kmax = tab->kmax; sa.dk = tab->dk;
     sa.n_orbit_blocks = (npv + kOrbitGroup - 1) / kOrbitGroup;
-    nblocks = sa.n_orbit_blocks + npv;
+    const long long nb = (long long)sa.n_orbit_blocks + (long long)npv * npb;
+    if (nb > 0x7fffffffLL) return set_error(RR_ERANGE, "%s: setup grid too large", who);
+    nblocks = (int)nb;
     return RR_OK;
 }
 
 // Validates one request and fills the flux kernel's arguments, its dynamic LDS
-// bytes and its variant (sorted single light curve or general).
+// bytes and its variant (light curves contiguous and sorted, or general).
 static int prepare_flux(const char* who, int precision, const rr_table* tab, const rr_lcdata* lc,
                         const rr_params* pv, void* workspace, size_t workspace_bytes, void* flux, FluxArgs& fa,
                         long long& nblocks, size_t& lds, bool& fast) {
@@ -849,7 +868,7 @@ static int prepare_flux(const char* who, int precision, const rr_table* tab, con
     nblocks = (long long)fa.ntiles * npv;
     if (nblocks > 0x7fffffffLL) return set_error(RR_ERANGE, "%s: grid too large", who);
     lds = flux_lds_bytes(mixed, tab->ng, fa.chunk_pts, fa.uniform_ns, fa.row_lds != 0);
-    fast = (nlc == 1) && lc->sorted;
+    fast = lc->sorted != 0;
     return RR_OK;
 }
 
@@ -897,7 +916,7 @@ int rr_evaluate_batch(int precision, const rr_table* tab, int nreq, const rr_req
         return set_error(RR_EINVAL, "rr_evaluate_batch: %d requests, at most %d per call", nreq, kMaxBatch);
     if (nreq > 0 && !reqs) return set_error(RR_EINVAL, "rr_evaluate_batch: null request array");
     SetupBatch sb;
-    FluxBatch fb[2];  // [0] general, [1] sorted single light curve
+    FluxBatch fb[2];  // [0] general, [1] contiguous sorted light curves
     size_t lds[2] = {0, 0};
     long long fblocks[2] = {0, 0};
     long long sblocks = 0;

@@ -36,7 +36,7 @@ import torch
 from . import _lib
 from ._lib import LD_LAW_IDS, RRLcData, RRLd, RRParams, RRRequest, check
 from .rrmodel import _PRECISIONS, RoadRunnerModel
-from .transitmodel import TransitModel
+from .transitmodel import TransitModel, runs_sorted
 
 __all__ = ['FreeList', 'Dataset', 'Request', 'RRServer']
 
@@ -117,7 +117,7 @@ class Dataset:
         self.npt, self.nlc, self.npb = tm.npt, tm.nlc, tm.npb
         self.nep = int(np.unique(tm.epids).size) if tm.epids.size else 1
         self.max_epid = int(tm.epids.max()) if tm.epids.size else 0
-        self.sorted = bool(tm.npt < 2 or np.all(np.diff(tm.time) >= 0))
+        self.sorted = runs_sorted(tm.time, tm.lcids)
         self.uniform_ns = int(nsam[0]) if (nsam == nsam[0]).all() else 0
         self.host = OrderedDict(times=np.ascontiguousarray(tm.time, np.float64),
                                 lcids=tm.lcids.astype(np.int32), pbids=tm.pbids.astype(np.int32),

@@ -13,6 +13,15 @@ import torch
 from numpy import asarray, atleast_1d, float64, integer, issubdtype, ndarray, ones, unique, zeros
 
 
+def runs_sorted(time: ndarray, lcids: ndarray) -> bool:
+    """True if the light curves are contiguous (lcids non-decreasing) and each is
+    sorted in time: the layout the flux kernel classifies run by run."""
+    if time.size < 2:
+        return True
+    dl = np.diff(lcids)
+    return bool(np.all(dl >= 0) and np.all(np.diff(time)[dl == 0] >= 0))
+
+
 class TransitModel:
     """Exoplanet transit light curve model (transitmodel.py:36-54)."""
 
@@ -113,7 +122,7 @@ class TransitModel:
             if (nsamples < 1).any():
                 raise ValueError('nsamples must be >= 1')
             d = dict(
-                sorted=bool(self.npt < 2 or np.all(np.diff(self.time) >= 0)),
+                sorted=runs_sorted(self.time, self.lcids),
                 uniform_ns=int(nsamples[0]) if (nsamples == nsamples[0]).all() else 0,
                 times=torch.as_tensor(np.ascontiguousarray(self.time), device=device),
                 lcids=torch.as_tensor(self.lcids.astype(np.int32), device=device),

@@ -160,6 +160,37 @@ def test_ld_edge_cases(oracle):
     assert oracle.ld_general(mu1, [0.5, 0.3])[0] == 0.0
 
 
+def test_remaining_laws_pinned_by_reductions_and_special_values(oracle):
+    """Laws the reference's tests do not cover (numba/ldmodels.py:88-139): each is
+    pinned by a reduction to an already-pinned law and by exact special values, so a
+    wrong sign, power or coefficient order fails."""
+    mu = np.linspace(0.01, 1.0, 40)
+    # square root: d = 0 is the linear law; c = 0 at mu = 1/4 gives 1 - d/2
+    np.testing.assert_allclose(oracle.ld_square_root(mu, [0.4, 0.0]), oracle.ld_linear(mu, [0.4]), rtol=1e-14)
+    assert oracle.ld_square_root(np.array([0.25]), [0.0, 0.6])[0] == pytest.approx(0.7, rel=1e-14)
+    # logarithmic: d = 0 is linear; c = 0 at mu = 1/e gives 1 + d/e
+    np.testing.assert_allclose(oracle.ld_logarithmic(mu, [0.4, 0.0]), oracle.ld_linear(mu, [0.4]), rtol=1e-14)
+    assert oracle.ld_logarithmic(np.array([math.exp(-1)]), [0.0, 0.5])[0] == pytest.approx(1 + 0.5 / math.e,
+                                                                                           rel=1e-14)
+    # exponential: d = 0 is linear; c = 0, d = 1 at mu = ln 2 gives 1 - 1/(1 - 2) = 2
+    np.testing.assert_allclose(oracle.ld_exponential(mu, [0.4, 0.0]), oracle.ld_linear(mu, [0.4]), rtol=1e-14)
+    assert oracle.ld_exponential(np.array([math.log(2.0)]), [0.0, 1.0])[0] == pytest.approx(2.0, rel=1e-14)
+    # power-2 in Maxted's (h1, h2) = (1 - c (1 - 2^-alpha), c 2^-alpha) equals power-2 in (c, alpha)
+    for c, alpha in ((0.6, 0.5), (0.8, 0.9), (0.3, 0.35)):
+        h1, h2 = 1 - c * (1 - 2 ** -alpha), c * 2 ** -alpha
+        np.testing.assert_allclose(oracle.ld_power_2_pm(mu, [h1, h2]), oracle.ld_power_2(mu, [c, alpha]),
+                                   rtol=1e-13)
+    # general (sum_i c_i (1 - mu^(i+1)), no leading 1): one coefficient is 1 - linear
+    np.testing.assert_allclose(oracle.ld_general(mu, [0.4]), 1.0 - oracle.ld_linear(mu, [0.4]), rtol=1e-14)
+    assert oracle.ld_general(np.array([0.5]), [0.0, 1.0, 0.0])[0] == pytest.approx(0.75, rel=1e-14)
+    # nonlinear: with only c2 it is linear; with only c4 it is quadratic-in-mu^2
+    np.testing.assert_allclose(oracle.ld_nonlinear(mu, [0.0, 0.4, 0.0, 0.0]), oracle.ld_linear(mu, [0.4]),
+                               rtol=1e-14)
+    assert oracle.ld_nonlinear(np.array([0.25]), [0.5, 0.0, 0.0, 0.0])[0] == pytest.approx(0.75, rel=1e-14)
+    assert oracle.ld_nonlinear(np.array([0.25]), [0.0, 0.0, 0.7, 0.0])[0] == pytest.approx(1 - 0.7 * 0.875,
+                                                                                          rel=1e-14)
+
+
 def test_trapezoid_istar_fallback(oracle):
     """Q8: laws without ldi use 2 pi trapz(z I, z) on mu = linspace(1, 0, 200)."""
     c = np.array([0.1, 0.2, 0.1, 0.05])

@@ -175,6 +175,57 @@ def test_heterogeneous_light_curves(gpu, oracle):
         assert err <= tol
 
 
+def test_contiguous_light_curves_run_classification(gpu, oracle):
+    say('case: 3 contiguous light curves (same times per band), per-lc supersampling, ragged lengths')
+    rng = np.random.default_rng(22)
+    t_lc = [np.linspace(-0.3, 0.3, n) for n in (997, 1003, 1500)]
+    times = np.concatenate(t_lc)
+    lcids = np.concatenate([np.full(t.size, j) for j, t in enumerate(t_lc)])
+    pbids, epids = np.array([0, 1, 2]), np.array([0, 1, 0])
+    nsamples, exptimes = np.array([1, 4, 7]), np.array([0.0, 0.01, 0.02])
+    npv = 24
+    k = rng.uniform(0.08, 0.12, (npv, 3))
+    t0 = rng.normal(0, 0.01, (npv, 2))
+    p = rng.uniform(1, 5, npv)
+    a = workloads.as_from_rhop(rng.uniform(1, 3, npv), p)
+    inc = workloads.i_from_ba(rng.uniform(0, 0.8, npv), a)
+    e, w = np.zeros(npv), np.zeros(npv)
+    ldc = rng.uniform(0.1, 0.4, (npv, 3, 2))
+    a[4] = 0.9
+    om = oracle.OracleRR('quadratic')
+    om.set_data(times, lcids=lcids, pbids=pbids, nsamples=nsamples, exptimes=exptimes, epids=epids)
+    fo = om.evaluate(k, ldc, t0, p, a, inc, e, w)
+    assert (fo < 1).sum() > 5000
+    for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
+        tm = RoadRunnerModel('quadratic', precision=prec)
+        tm.set_data(times, lcids=lcids, pbids=pbids, nsamples=nsamples, exptimes=exptimes, epids=epids)
+        assert tm._device_data(tm.device)['sorted']
+        fg = np.asarray(tm.evaluate(k, ldc, t0, p, a, inc, e, w), float)
+        assert np.array_equal(np.isnan(fo), np.isnan(fg)), prec
+        err = np.nanmax(np.abs(fg - fo))
+        say(f'  {prec}: max |dF| = {err:.3e}')
+        assert err <= tol
+
+
+def test_bjd_segments_many_passbands(gpu, oracle):
+    say('case: BJD 2-min cadence split into 7 contiguous passband light curves, nss=5, power-2')
+    wl = workloads.rr_bjd(npv=32, npt=6000, seed=9)
+    wl.nsamples = 5
+    npb = 7
+    lcids = (np.arange(wl.times.size) * npb) // wl.times.size
+    rng = np.random.default_rng(9)
+    wl.k = wl.k + 0.002 * np.sin(np.arange(npb) / 3.0)[None, :]
+    wl.ldc = np.concatenate([wl.ldc] * npb, 1) + rng.uniform(-0.05, 0.05, (32, npb, 2))
+    pbids = np.arange(npb)
+    fo = oracle_flux(oracle, wl, 'power-2', lcids=lcids, pbids=pbids)
+    assert (fo < 1).sum() > 1000
+    for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
+        fg = gpu_flux(wl, prec, lcids=lcids, pbids=pbids)
+        err = np.nanmax(np.abs(fg - fo))
+        say(f'  {prec}: max |dF| = {err:.3e}')
+        assert err <= tol
+
+
 def test_k_outside_table_uses_exact_weights(gpu, oracle):
     say('case: k outside klims -> exact 2-D weights')
     wl = workloads.rr_small(npv=8)
