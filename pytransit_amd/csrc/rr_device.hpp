@@ -19,9 +19,34 @@ constexpr float kPiF = 3.14159265358979323846f;
 
 // ---- overloads so templates read the same for float and double -----------------------
 __device__ __forceinline__ double d_sqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float d_sqrt(float x) { return __builtin_sqrtf(x); }
-__device__ __forceinline__ double d_atan2(double y, double x) { return atan2(y, x); }
-__device__ __forceinline__ float d_atan2(float y, float x) { return atan2f(y, x); }
+// FP32: the hardware square root (1 ulp) without sqrtf's denormal rescaling; the
+// arguments here are squared distances and kite products, never denormal.
+__device__ __forceinline__ float d_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// atan2(y, x) for y >= 0 in FP32: octant reduction with one hardware reciprocal
+// and atan(t) = t P(t^2) on [0, 1], P of degree 7 fitted by Lawson's minimax
+// iteration (pytransit_amd/csrc/atan_poly.py).  Max error 1.4e-7 rad.
+__device__ __forceinline__ float atan2_ypos_f(float y, float x) {
+    const float ax = fabsf(x);
+    const float mx = fmaxf(ax, y), mn = fminf(ax, y);
+    const float t = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+    const float s = t * t;
+    float p = -0.00405456312f;
+    p = fmaf(p, s, 0.0218629446f);
+    p = fmaf(p, s, -0.0559123158f);
+    p = fmaf(p, s, 0.0964219719f);
+    p = fmaf(p, s, -0.139086291f);
+    p = fmaf(p, s, 0.199465662f);
+    p = fmaf(p, s, -0.333298594f);
+    p = fmaf(p, s, 0.999999344f);
+    float r = t * p;
+    r = (y > ax) ? 1.57079633f - r : r;
+    return (x < 0.0f) ? 3.14159265f - r : r;
+}
+
+// atan2 with y >= 0 (the kite angles): libm in FP64, the polynomial in FP32.
+__device__ __forceinline__ double d_atan2_ypos(double y, double x) { return atan2(y, x); }
+__device__ __forceinline__ float d_atan2_ypos(float y, float x) { return atan2_ypos_f(y, x); }
 __device__ __forceinline__ double d_abs(double x) { return fabs(x); }
 __device__ __forceinline__ float d_abs(float x) { return fabsf(x); }
 __device__ __forceinline__ double d_max(double a, double b) { return fmax(a, b); }
@@ -65,8 +90,8 @@ __device__ __forceinline__ void ccia_kite3(T r1, T r2, T b, T& area, T& kap0, T&
         const T z = d_min(lo12, b);
         const T y = d_max(lo12, d_min(hi12, b));
         const T akite = T(0.5) * d_sqrt((x + (y + z)) * (z - (x - y)) * (z + (x - y)) * (x + (y - z)));
-        const T k0 = d_atan2(T(2) * akite, (r2 - r1) * (r2 + r1) + b * b);
-        const T k1 = d_atan2(T(2) * akite, (r1 - r2) * (r1 + r2) + b * b);
+        const T k0 = d_atan2_ypos(T(2) * akite, (r2 - r1) * (r2 + r1) + b * b);
+        const T k1 = d_atan2_ypos(T(2) * akite, (r1 - r2) * (r1 + r2) + b * b);
         area = r1 * r1 * k1 + r2 * r2 * k0 - akite;
         kap0 = k0;
         akite_out = akite;

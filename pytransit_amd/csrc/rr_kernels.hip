@@ -560,35 +560,34 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     }
     __syncthreads();  // queue entries and the staged row are read by other lanes
     const float inv_dg = (float)(1.0 / f.dg);
-    if (f.uniform_ns > 0 && f.nlc == 1) {
+    if (f.uniform_ns > 0 && f.nlc == 1 && f.row_lds) {
         // one light curve: the per-point context is uniform
         const int ns = f.uniform_ns;
         const PointCtx c = point_ctx(f, r, pv, 0);
-        const size_t row = ((size_t)pv * f.npb + c.ipb) * f.ng;
         const float inv_ns = 1.0f / (float)ns;
         const float kf = (float)c.kk, extf = (float)c.ext;
         float cf[10];
 #pragma unroll
         for (int k = 0; k < 10; ++k) cf[k] = (float)r.c[k];
+        float* s_tcf = (float*)s_tc;
         const int cp = f.chunk_pts;
         for (int c0 = 0; c0 < n; c0 += cp) {
             const int cn = min(cp, n - c0);
             if (wl < cn) {
                 double ep;
-                s_tc[wl] = phase_of(f.times[tbase + s_q[c0 + wl]], c.t0v, p, inv_p, ep);
+                const double tc = phase_of(f.times[tbase + s_q[c0 + wl]], c.t0v, p, inv_p, ep);
+                if (kMixed) s_tcf[wl] = (float)tc;
+                else s_tc[wl] = tc;
             }
             __syncthreads();
 #pragma unroll 1
             for (int i = wl; i < cn * ns; i += kThreads) {
                 const int qq = (int)(((float)i + 0.5f) * inv_ns);  // exact i / ns for i, ns <= 1024
                 const int is = i - qq * ns;
-                if (kMixed) {
-                    const float* lrow = f.row_lds ? (const float*)s_row : f.ldm32 + row;
-                    s_dep[i] = sample_f32(cf, kf, inv_dg, (float)s_tc[qq], extf, inv_ns, lrow, f.ng, is);
-                } else {
-                    const double* lrow = f.row_lds ? (const double*)s_row : f.ldm64 + row;
-                    s_dep[i] = sample_f64(r.c, f.dg, f.ng, c, lrow, s_tc[qq], is);
-                }
+                if (kMixed)
+                    s_dep[i] = sample_f32(cf, kf, inv_dg, s_tcf[qq], extf, inv_ns, (const float*)s_row, f.ng, is);
+                else
+                    s_dep[i] = sample_f64(r.c, f.dg, f.ng, c, (const double*)s_row, s_tc[qq], is);
             }
             __syncthreads();
             if (wl < cn) {
