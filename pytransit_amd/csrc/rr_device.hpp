@@ -50,9 +50,10 @@ __device__ __forceinline__ float d_atan2_ypos(float y, float x) { return atan2_y
 __device__ __forceinline__ double d_abs(double x) { return fabs(x); }
 __device__ __forceinline__ float d_abs(float x) { return fabsf(x); }
 __device__ __forceinline__ double d_max(double a, double b) { return fmax(a, b); }
-__device__ __forceinline__ float d_max(float a, float b) { return fmaxf(a, b); }
+// FP32 min/max as selects: no NaN canonicalisation (the FP32 geometry never sees NaN).
+__device__ __forceinline__ float d_max(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ double d_min(double a, double b) { return fmin(a, b); }
-__device__ __forceinline__ float d_min(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float d_min(float a, float b) { return a < b ? a : b; }
 template <typename T> __device__ __forceinline__ T pi_of();
 template <> __device__ __forceinline__ double pi_of<double>() { return kPi; }
 template <> __device__ __forceinline__ float pi_of<float>() { return kPiF; }

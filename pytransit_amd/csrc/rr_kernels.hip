@@ -10,8 +10,9 @@
 //
 // Setup is one launch with two block roles: orbit blocks (8 pv per 64-lane
 // block, the 7 Kepler solves of a pv on 7 lanes) and limb-darkening blocks (one
-// (pv, passband) pair per block: profile at the mu nodes in LDS, disk integral,
-// and the table-lerp contraction with lanes over g reading the table coalesced).
+// (pv, group of up to 8 passbands) per block: profiles at the mu nodes in LDS, disk
+// integrals, and the table-lerp contraction with lanes over g reading the table
+// coalesced, each table value serving the whole group).
 //
 // The flux kernel runs one wavefront per workgroup over a tile of 1024 times of
 // one pv; each lane owns two runs of 8 consecutive times and issues all its time
@@ -198,12 +199,26 @@ __device__ void setup_orbit_role(const SetupArgs& s, int blk) {
     }
 }
 
-// One (pv, passband) pair per block: the profile at the mu nodes, its disk
-// integral and the limb-darkening means of that passband.
-__device__ void setup_ld_role(const SetupArgs& s, int pv, int ipb) {
-    __shared__ double s_l[kMaxNz];
+// One block per (pv, group of up to kLdGroup passbands): the profiles at the mu
+// nodes, their disk integrals and the limb-darkening means.  The passbands of a
+// group share each table read (every passband of a pv uses the rows of k0,
+// model_full.py:47-51).
+constexpr int kLdGroup = 8;
+constexpr int kLdLds = 1024;  // doubles of profile storage per block
+
+__host__ __device__ inline int ld_group_size(int nz) {
+    const int g = kLdLds / (nz > 0 ? nz : 1);
+    return g < kLdGroup ? (g < 1 ? 1 : g) : kLdGroup;
+}
+
+__device__ void setup_ld_role(const SetupArgs& s, int pv, int grp) {
+    __shared__ double s_l[kLdLds];
+    __shared__ double s_ist[kLdGroup];
     const int lane = threadIdx.x;
     const int nz = s.nz, ng = s.ng;
+    const int gs = ld_group_size(nz);
+    const int pb0 = grp * gs;
+    const int gn = min(gs, s.npb - pb0);
     const double k0 = s.k[(size_t)pv * s.kcols];
     const bool in_table = s.W != nullptr && s.kmin <= k0 && k0 <= s.kmax;
     int ik = 0;
@@ -214,48 +229,57 @@ __device__ void setup_ld_role(const SetupArgs& s, int pv, int ipb) {
         ak = (k0 - s.kmin - ik * s.dk) / s.dk;
     }
     const bool mixed = s.ldm32 != nullptr;
-    const size_t j = (size_t)pv * s.npb + ipb;
-    // profile at the mu nodes and its disk integral
-    double ist;
-    if (s.law == RR_LD_TABULATED) {
-        for (int iz = lane; iz < nz; iz += 64) s_l[iz] = s.ldp_in[j * nz + iz];
-        ist = s.istar_in[j];
-    } else {
-        const double* c = s.ldc + j * s.nldc;
-        for (int iz = lane; iz < nz; iz += 64) s_l[iz] = ld_law(s.law, s.mu[iz], c, s.nldc);
-        if (!ld_integral(s.law, c, ist)) {
-            double part = 0.0;
-            for (int m = 1 + lane; m < 200; m += 64) part += ld_trapz_segment(s.law, c, s.nldc, m);
-            ist = 2.0 * kPi * wave_sum(part);
+    const size_t j0 = (size_t)pv * s.npb + pb0;
+    // profiles at the mu nodes and their disk integrals
+    for (int q = 0; q < gn; ++q) {
+        const size_t j = j0 + q;
+        double* l = s_l + q * nz;
+        double ist;
+        if (s.law == RR_LD_TABULATED) {
+            for (int iz = lane; iz < nz; iz += 64) l[iz] = s.ldp_in[j * nz + iz];
+            ist = s.istar_in[j];
+        } else {
+            const double* c = s.ldc + j * s.nldc;
+            for (int iz = lane; iz < nz; iz += 64) l[iz] = ld_law(s.law, s.mu[iz], c, s.nldc);
+            if (!ld_integral(s.law, c, ist)) {
+                double part = 0.0;
+                for (int m = 1 + lane; m < 200; m += 64) part += ld_trapz_segment(s.law, c, s.nldc, m);
+                ist = 2.0 * kPi * wave_sum(part);
+            }
         }
+        if (lane == 0) s_ist[q] = ist;
     }
     __syncthreads();
-    if (lane == 0) {
-        s.istar[j] = ist;
-        if (ipb == 0) s.rec[pv].ldok = isnan(s_l[0]) ? 0 : 1;
-    }
+    if (lane < gn) s.istar[j0 + lane] = s_ist[lane];
+    if (lane == 0 && pb0 == 0) s.rec[pv].ldok = isnan(s_l[0]) ? 0 : 1;
     // limb-darkening means: lerp in k of two table rows, or exact 2-D weights
     // when k0 is outside [kmin, kmax]; model_full.py:47-55
     for (int ig = lane; ig < ng; ig += 64) {
-        double v;
+        double v[kLdGroup];
         if (in_table) {
-            double d0 = 0.0, d1 = 0.0;
-            if (mixed && s.W32) {
-                const float* w0 = s.W32 + (size_t)ik * nz * ng + ig;
-                const float* w1 = w0 + (size_t)nz * ng;
-                for (int iz = 0; iz < nz; ++iz) {
-                    d0 += (double)w0[(size_t)iz * ng] * s_l[iz];
-                    d1 += (double)w1[(size_t)iz * ng] * s_l[iz];
+            double d0[kLdGroup], d1[kLdGroup];
+#pragma unroll
+            for (int q = 0; q < kLdGroup; ++q) d0[q] = d1[q] = 0.0;
+            const size_t o0 = (size_t)ik * nz * ng + ig, o1 = o0 + (size_t)nz * ng;
+            for (int iz = 0; iz < nz; ++iz) {
+                double w0, w1;
+                if (mixed && s.W32) {
+                    w0 = (double)s.W32[o0 + (size_t)iz * ng];
+                    w1 = (double)s.W32[o1 + (size_t)iz * ng];
+                } else {
+                    w0 = s.W[o0 + (size_t)iz * ng];
+                    w1 = s.W[o1 + (size_t)iz * ng];
                 }
-            } else {
-                const double* w0 = s.W + (size_t)ik * nz * ng + ig;
-                const double* w1 = w0 + (size_t)nz * ng;
-                for (int iz = 0; iz < nz; ++iz) {
-                    d0 += w0[(size_t)iz * ng] * s_l[iz];
-                    d1 += w1[(size_t)iz * ng] * s_l[iz];
+#pragma unroll
+                for (int q = 0; q < kLdGroup; ++q) {
+                    if (q < gn) {
+                        d0[q] += w0 * s_l[q * nz + iz];
+                        d1[q] += w1 * s_l[q * nz + iz];
+                    }
                 }
             }
-            v = (1.0 - ak) * d0 + ak * d1;
+#pragma unroll
+            for (int q = 0; q < kLdGroup; ++q) v[q] = (1.0 - ak) * d0[q] + ak * d1[q];
         } else {
             // common.py:152-185: normalised annulus areas, recomputed in a second pass
             const double g = linspace_at(0.0, 1.0 - 1e-7, ng, ig);
@@ -267,15 +291,24 @@ __device__ void setup_ld_role(const SetupArgs& s, int pv, int ipb) {
                 a0 = a1;
             }
             double b0 = cci_area(s.ze[0], k0, b);
-            v = (b0 / sum) * s_l[0];
+#pragma unroll
+            for (int q = 0; q < kLdGroup; ++q) v[q] = (b0 / sum) * s_l[(q < gn ? q : 0) * nz];
             for (int iz = 1; iz < nz; ++iz) {
                 const double b1 = cci_area(s.ze[iz], k0, b);
-                v += ((b1 - b0) / sum) * s_l[iz];
+                const double wv = (b1 - b0) / sum;
+#pragma unroll
+                for (int q = 0; q < kLdGroup; ++q)
+                    if (q < gn) v[q] += wv * s_l[q * nz + iz];
                 b0 = b1;
             }
         }
-        if (mixed) s.ldm32[j * ng + ig] = (float)v;
-        else s.ldm64[j * ng + ig] = v;
+#pragma unroll
+        for (int q = 0; q < kLdGroup; ++q) {
+            if (q < gn) {
+                if (mixed) s.ldm32[(j0 + q) * ng + ig] = (float)v[q];
+                else s.ldm64[(j0 + q) * ng + ig] = v[q];
+            }
+        }
     }
 }
 
@@ -284,8 +317,9 @@ __device__ __forceinline__ void setup_body(const SetupArgs& s, int blk) {
         setup_orbit_role(s, blk);
     } else {
         const int b = blk - s.n_orbit_blocks;
-        const int pv = b / s.npb;
-        setup_ld_role(s, pv, b - pv * s.npb);
+        const int ngrp = (s.npb + ld_group_size(s.nz) - 1) / ld_group_size(s.nz);
+        const int pv = b / ngrp;
+        setup_ld_role(s, pv, b - pv * ngrp);
     }
 }
 
@@ -372,6 +406,12 @@ __device__ __forceinline__ double phase_of(double t, double t0, double p, double
     ep += (r < 0.0) ? -1.0 : ((r >= p) ? 1.0 : 0.0);
     epoch = ep;
     return t - (t0 + ep * p);
+}
+
+// t0 + epoch * p, the subtrahend of model_full.py:89, rounded as the reference does.
+__device__ __forceinline__ double epoch_base(double t0, double epoch, double p) {
+#pragma clang fp contract(off)
+    return t0 + epoch * p;
 }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -493,7 +533,8 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
         const PointCtx c1 = point_ctx(f, r, pv, 0);
 #pragma unroll
         for (int m = 0; m < kRuns; ++m) {
-            bool all_out = true, one_lc = true;
+            bool all_out = true, one_lc = true, one_ep = false;
+            double base = 0.0;
             PointCtx c = c1;
             if (nrun[m] > 0 && f.nlc > 1) {
                 const int la = min(max(f.lcids[its[m]], 0), f.nlc - 1);
@@ -509,6 +550,9 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                     const double tcl = phase_of(tl[m], c.t0v, p, inv_p, el);
                     all_out = (ef == el) ? (tcl < c.lo || tcf > c.hi)
                                          : ((el == ef + 1.0) ? (tcf > c.hi && tcl < c.lo) : false);
+                    // the epoch is monotone in t: equal at both ends, equal throughout
+                    one_ep = ef == el;
+                    base = epoch_base(c.t0v, ef, p);
                 }
             }
             const int step0 = tbase + m * kThreads * kRun;
@@ -522,9 +566,15 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                 bool inw = false;
                 const int it = its[m] + j;
                 if (!all_out && j < nrun[m]) {
-                    const PointCtx cc = one_lc ? c : point_ctx(f, r, pv, min(max(f.lcids[it], 0), f.nlc - 1));
-                    double ep;
-                    const double tc = phase_of(f.times[it], cc.t0v, p, inv_p, ep);
+                    double tc;
+                    PointCtx cc = c;
+                    if (one_ep) {
+                        tc = f.times[it] - base;  // model_full.py:89 with the run's epoch
+                    } else {
+                        if (!one_lc) cc = point_ctx(f, r, pv, min(max(f.lcids[it], 0), f.nlc - 1));
+                        double ep;
+                        tc = phase_of(f.times[it], cc.t0v, p, inv_p, ep);
+                    }
                     inw = (cc.lo <= tc) && (tc <= cc.hi);
                     if (!inw) out[it] = Real(1);
                 }
@@ -832,7 +882,8 @@ static int prepare_setup(const char* who, int precision, const rr_table* tab, co
     sa.npv = npv; sa.npb = npb; sa.kcols = pv->kcols; sa.nk = tab->nk; sa.ng = tab->ng; sa.nz = tab->nz;
     sa.kmin = tab->kmin; sa.kmax = tab->kmax; sa.dk = tab->dk;
     sa.n_orbit_blocks = (npv + kOrbitGroup - 1) / kOrbitGroup;
-    const long long nb = (long long)sa.n_orbit_blocks + (long long)npv * npb;
+    const int gs = ld_group_size(tab->nz);
+    const long long nb = (long long)sa.n_orbit_blocks + (long long)npv * ((npb + gs - 1) / gs);
     if (nb > 0x7fffffffLL) return set_error(RR_ERANGE, "%s: setup grid too large", who);
     nblocks = (int)nb;
     return RR_OK;

@@ -14,9 +14,11 @@
 //                      transposed W2T[pv][z][g] (FP64 and FP32, g padded to 16)
 //   ts_ldm_mfma_kernel FP32 MFMA contraction ldm = ldp . W2T, one workgroup per pv:
 //                      v_mfma_f32_16x16x4f32 tiles, W2T in LDS, profile rows streamed
-//   ts_ldm_f64_kernel  the FP64 contraction on the vector ALUs
-//   ts_flux_kernel     one lane per time, a workgroup per (pv, 256 times, 64|32
-//                      passbands): geometry in FP64, passband fan-out from LDS
+//   ts_ldm_mfma64_kernel the FP64 contraction on v_mfma_f64_16x16x4f64 (W2T in LDS
+//                      when it fits in 128 KB), else ts_ldm_f64_kernel on the VALU
+//   ts_flux_kernel     one lane per time, a workgroup per (pv, 256 times): the FP64
+//                      geometry once per (time, sample), then the passband fan-out
+//                      over chunks of 64 (FP32) or 32 (FP64) LD rows staged in LDS
 //
 // Flux layout [npv, npb, npt]; for one passband row 256 lanes write 1 KB (FP32)
 // contiguously.
@@ -253,6 +255,61 @@ __global__ void __launch_bounds__(256) ts_ldm_mfma_kernel(TsArgs s) {
     }
 }
 
+// ---- FP64 MFMA contraction (v_mfma_f64_16x16x4f64), same tiling as the FP32 one ---------
+// Lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds D[(l/16)+4r][l%16]:
+// unlike the FP32 16x16x4 form, the four result rows of a lane are interleaved.
+// One workgroup of 8 waves per pv; W2T (FP64) sits in LDS with a padded row stride.
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+constexpr int kMfma64Threads = 512;
+constexpr size_t kMfma64MaxLds = 128 * 1024;
+
+__global__ void __launch_bounds__(kMfma64Threads) ts_ldm_mfma64_kernel(TsArgs s) {
+    extern __shared__ double lds_b64[];  // [nzp][ngp + 1]
+    const int pv = blockIdx.x;
+    const int ldb = s.ngp + 1;
+    for (int i = threadIdx.x; i < s.nzp * s.ngp; i += blockDim.x) {
+        const int iz = i / s.ngp, ig = i - iz * s.ngp;
+        lds_b64[iz * ldb + ig] = s.w2_64[(size_t)pv * s.nzp * s.ngp + i];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int nrt = (s.npb + 15) / 16, nct = s.ngp / 16;
+    constexpr int kWaves = kMfma64Threads / 64;
+    for (int rt = wave; rt < nrt; rt += kWaves) {
+        const int pb = rt * 16 + li;
+        const bool row_ok = pb < s.npb;
+        for (int ct0 = 0; ct0 < nct; ct0 += 8) {
+            f64x4_t acc[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) acc[c] = (f64x4_t){0.0, 0.0, 0.0, 0.0};
+            for (int k0 = 0; k0 < s.nzp; k0 += 4) {
+                const int iz = k0 + lk;
+                const double a = (row_ok && iz < s.nz) ? ts_profile(s, pv, pb, iz) : 0.0;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int ct = ct0 + c;
+                    if (ct < nct) {
+                        const double b = lds_b64[iz * ldb + ct * 16 + li];
+                        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int ct = ct0 + c;
+                if (ct >= nct) break;
+                const int g = ct * 16 + li;
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
+                    const int pbo = rt * 16 + lk + 4 * rr;
+                    if (pbo < s.npb && g < s.ng) s.ldm64[((size_t)pv * s.npb + pbo) * s.ng + g] = acc[c][rr];
+                }
+            }
+        }
+    }
+}
+
 // ---- FP64 contraction on the vector ALUs: one lane per (pv, pb, g) --------------------
 __global__ void __launch_bounds__(256) ts_ldm_f64_kernel(TsArgs s) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -269,35 +326,74 @@ __global__ void __launch_bounds__(256) ts_ldm_f64_kernel(TsArgs s) {
 // ---- flux --------------------------------------------------------------------------
 constexpr int kTsThreads = 256;
 
-template <bool kMixed>
-struct TsCfg {
-    static constexpr int kPbChunk = kMixed ? 64 : 32;
+// Passbands per LDS chunk: 64 (FP32) / 32 (FP64) with one accumulator per lane,
+// half that when a lane keeps one accumulator per passband of the chunk.
+constexpr int ts_pb_chunk(bool mixed, bool cached) { return cached ? (mixed ? 64 : 32) : (mixed ? 32 : 16); }
+
+// Geometry of one (time, sample): the lens at k_mean in FP64 (model_trspec.py:79-84)
+// and the lerp position in g; shared by every passband.
+template <typename Real>
+struct TsGeo {
+    Real ra, rb;      // lerp weights of l[ig + 1] and l[ig]
+    Real ap0, dadk;   // area at k_mean and its k-derivative 2 k_mean kappa_0
+    int ig;
+    bool inside;      // z <= 1 - k_max: area scales as k^2 / k_mean^2
+    bool gz;          // g > 1: the limb-darkening mean is 0
 };
 
-template <bool kMixed>
-__global__ void __launch_bounds__(kTsThreads) ts_flux_kernel(TsArgs s, void* flux, int ntiles, int nchunks) {
+template <typename Real>
+__device__ __forceinline__ TsGeo<Real> ts_geometry(const TsRec& r, double tc, double ext, int ns, int is,
+                                                   double dg, int ng) {
+    TsGeo<Real> o;
+    const double km = r.kmean;
+    const double off = ext * ((is + 1 - 0.5) / ns - 0.5);
+    const double z = taylor_z<double>(tc + off, r.c);
+    double ap0, kap;
+    ccia_kite<double>(1.0, km, z, ap0, kap);
+    o.inside = z <= 1.0 - r.kmax;
+    const double g = z / (1.0 + km);
+    // lerp position (common.py:225-233, index clamped to ng-2)
+    o.gz = g > 1.0;
+    o.ig = 0;
+    double al = 0.0;
+    if (!o.gz) {
+        int ig = (int)floor(g / dg);
+        o.ig = ig > ng - 2 ? ng - 2 : (ig < 0 ? 0 : ig);
+        al = (g - o.ig * dg) / dg;
+    }
+    o.ra = (Real)al;
+    o.rb = (Real)(1.0 - al);
+    o.ap0 = (Real)ap0;
+    o.dadk = (Real)(2.0 * km * kap);
+    return o;
+}
+
+// With nsamples <= kTsGeoRegs the geometry of a time's samples stays in registers
+// and the passbands are the outer loop (one accumulator); otherwise the samples
+// are the outer loop over a chunk of PC accumulators and the geometry is
+// recomputed per chunk.
+constexpr int kTsGeoRegs = 2;
+
+template <bool kMixed, bool kCached>
+__global__ void __launch_bounds__(kTsThreads) ts_flux_kernel(TsArgs s, void* flux, int ntiles) {
     using Real = typename std::conditional<kMixed, float, double>::type;
-    constexpr int PC = TsCfg<kMixed>::kPbChunk;
+    constexpr int PC = ts_pb_chunk(kMixed, kCached);
     extern __shared__ __align__(16) unsigned char lds[];
     Real* s_ldm = (Real*)lds;                       // [PC][ng]
     Real* s_afac = s_ldm + (size_t)PC * s.ng;       // k^2 / k_mean^2
     Real* s_kd = s_afac + PC;                       // k - k_mean
-    double* s_ist = (double*)(s_kd + PC + (kMixed ? (PC & 1) : 0));
+    Real* s_iist = s_kd + PC;                       // 1 / istar
     __shared__ int s_any;
 
-    const int b = blockIdx.x;
-    const int chunk = b % nchunks;
-    const int tile = (b / nchunks) % ntiles;
-    const int pv = b / (nchunks * ntiles);
-    const int pb0 = chunk * PC;
-    const int npc = min(PC, s.npb - pb0);
+    const int tile = blockIdx.x % ntiles;
+    const int pv = blockIdx.x / ntiles;
     const int t = tile * kTsThreads + threadIdx.x;
     const bool tok = t < s.npt;
     const TsRec& r = s.rec[pv];
-    Real* out = (Real*)flux + ((size_t)pv * s.npb + pb0) * s.npt;
+    Real* out = (Real*)flux + (size_t)pv * s.npb * s.npt;
     if (!r.good) {
         if (tok)
-            for (int q = 0; q < npc; ++q) out[(size_t)q * s.npt + t] = Real(NAN);
+            for (int q = 0; q < s.npb; ++q) out[(size_t)q * s.npt + t] = Real(NAN);
         return;
     }
     double tc = 0.0;
@@ -318,71 +414,70 @@ __global__ void __launch_bounds__(kTsThreads) ts_flux_kernel(TsArgs s, void* flu
     __syncthreads();
     if (!s_any) {
         if (tok)
-            for (int q = 0; q < npc; ++q) out[(size_t)q * s.npt + t] = Real(1);
+            for (int q = 0; q < s.npb; ++q) out[(size_t)q * s.npt + t] = Real(1);
         return;
     }
-    // stage the chunk's limb-darkening means and per-passband factors
-    const double km = r.kmean;
-    for (int i = threadIdx.x; i < npc * s.ng; i += kTsThreads) {
-        const size_t src = ((size_t)pv * s.npb + pb0) * s.ng + i;
-        s_ldm[i] = kMixed ? (Real)s.ldm32[src] : (Real)s.ldm64[src];
-    }
-    for (int q = threadIdx.x; q < npc; q += kTsThreads) {
-        const double kq = s.k[(size_t)pv * s.npb + pb0 + q];
-        s_afac[q] = (Real)(kq * kq / (km * km));
-        s_kd[q] = (Real)(kq - km);
-        s_ist[q] = s.istar[(size_t)pv * s.npb + pb0 + q];
-    }
-    __syncthreads();
-
-    Real acc[PC];
-#pragma unroll
-    for (int q = 0; q < PC; ++q) acc[q] = Real(0);
     const int ns = s.nsamples[0];
     const double ext = s.exptimes[0];
-    if (inw) {
-        for (int is = 1; is <= ns; ++is) {
-            // geometry in FP64 once per (time, sample); model_trspec.py:79-84
-            const double off = ext * ((is - 0.5) / ns - 0.5);
-            const double z = taylor_z<double>(tc + off, r.c);
-            double ap0, kap;
-            ccia_kite<double>(1.0, km, z, ap0, kap);
-            const double dadk = 2.0 * km * kap;
-            const bool inside = z <= 1.0 - r.kmax;
-            const double g = z / (1.0 + km);
-            // lerp position (common.py:225-233, index clamped to ng-2)
-            bool gz = g > 1.0;
-            int ig = 0;
-            double al = 0.0;
-            if (!gz) {
-                ig = (int)floor(g / s.dg);
-                ig = ig > s.ng - 2 ? s.ng - 2 : (ig < 0 ? 0 : ig);
-                al = (g - ig * s.dg) / s.dg;
-            }
-            const Real ra = (Real)al, rb = (Real)(1.0 - al);
-            const Real rap0 = (Real)ap0, rdadk = (Real)dadk;
+    const Real inv_ns = Real(1) / Real(ns);
+    const double km = r.kmean;
+    TsGeo<Real> geo[kCached ? kTsGeoRegs : 1];
+    if (kCached) {
 #pragma unroll
-            for (int q = 0; q < PC; ++q) {
-                if (q < npc) {
-                    const Real* l = s_ldm + (size_t)q * s.ng;
-                    const Real ip = gz ? Real(0) : (rb * l[ig] + ra * l[ig + 1]);
-                    const Real area = inside ? rap0 * s_afac[q] : rap0 + s_kd[q] * rdadk;
-                    if (kMixed) acc[q] += ip * area;                                   // depth
-                    else acc[q] += (s_ist[q] - ip * area) / s_ist[q];                   // reference form
+        for (int is = 0; is < kTsGeoRegs; ++is)
+            if (inw && is < ns) geo[is] = ts_geometry<Real>(r, tc, ext, ns, is, s.dg, s.ng);
+    }
+
+    for (int pb0 = 0; pb0 < s.npb; pb0 += PC) {
+        const int npc = min(PC, s.npb - pb0);
+        __syncthreads();  // the previous chunk's rows are no longer read
+        // stage the chunk's limb-darkening means and per-passband factors
+        for (int i = threadIdx.x; i < npc * s.ng; i += kTsThreads) {
+            const size_t src = ((size_t)pv * s.npb + pb0) * s.ng + i;
+            s_ldm[i] = kMixed ? (Real)s.ldm32[src] : (Real)s.ldm64[src];
+        }
+        for (int q = threadIdx.x; q < npc; q += kTsThreads) {
+            const double kq = s.k[(size_t)pv * s.npb + pb0 + q];
+            s_afac[q] = (Real)(kq * kq / (km * km));
+            s_kd[q] = (Real)(kq - km);
+            s_iist[q] = (Real)(1.0 / s.istar[(size_t)pv * s.npb + pb0 + q]);
+        }
+        __syncthreads();
+        if (!tok) continue;
+        Real* o = out + (size_t)pb0 * s.npt + t;
+        // depth of passband q for one sample; the flux is 1 - mean(depth) / istar
+        auto depth = [&](const TsGeo<Real>& g, int q) {
+            const Real* l = s_ldm + (size_t)q * s.ng;
+            const Real ip = g.gz ? Real(0) : (g.rb * l[g.ig] + g.ra * l[g.ig + 1]);
+            const Real area = g.inside ? g.ap0 * s_afac[q] : g.ap0 + s_kd[q] * g.dadk;
+            return ip * area;
+        };
+        if (kCached) {
+            if (!inw) {
+                for (int q = 0; q < npc; ++q) o[(size_t)q * s.npt] = Real(1);
+                continue;
+            }
+#pragma unroll 4
+            for (int q = 0; q < npc; ++q) {
+                Real acc = depth(geo[0], q);
+                if (ns > 1) acc += depth(geo[kCached ? kTsGeoRegs - 1 : 0], q);
+                o[(size_t)q * s.npt] = Real(1) - acc * (s_iist[q] * inv_ns);
+            }
+        } else {
+            Real acc[PC];
+#pragma unroll
+            for (int q = 0; q < PC; ++q) acc[q] = Real(0);
+            if (inw) {
+                for (int is = 0; is < ns; ++is) {
+                    const TsGeo<Real> g = ts_geometry<Real>(r, tc, ext, ns, is, s.dg, s.ng);
+#pragma unroll
+                    for (int q = 0; q < PC; ++q)
+                        if (q < npc) acc[q] += depth(g, q);
                 }
             }
-        }
-    }
-    if (tok) {
 #pragma unroll
-        for (int q = 0; q < PC; ++q) {
-            if (q < npc) {
-                Real v;
-                if (!inw) v = Real(1);
-                else if (kMixed) v = Real(1.0f - (float)acc[q] * (float)(1.0 / (s_ist[q] * ns)));
-                else v = Real(acc[q] / ns);
-                out[(size_t)q * s.npt + t] = v;
-            }
+            for (int q = 0; q < PC; ++q)
+                if (q < npc) o[(size_t)q * s.npt] = inw ? Real(1) - acc[q] * (s_iist[q] * inv_ns) : Real(1);
         }
     }
 }
@@ -470,25 +565,32 @@ int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
         hipLaunchKernelGGL(ts_ldm_mfma_kernel, dim3(npv), dim3(256), lds, st, a);
         if ((rc = check_launch("ts_ldm_mfma_kernel"))) return rc;
     } else {
-        const long long n = (long long)npv * npb * a.ng;
-        hipLaunchKernelGGL(ts_ldm_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
-        if ((rc = check_launch("ts_ldm_f64_kernel"))) return rc;
+        const size_t lds = (size_t)a.nzp * (a.ngp + 1) * sizeof(double);
+        if (lds <= kMfma64MaxLds) {
+            hipLaunchKernelGGL(ts_ldm_mfma64_kernel, dim3(npv), dim3(kMfma64Threads), lds, st, a);
+            if ((rc = check_launch("ts_ldm_mfma64_kernel"))) return rc;
+        } else {
+            const long long n = (long long)npv * npb * a.ng;
+            hipLaunchKernelGGL(ts_ldm_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+            if ((rc = check_launch("ts_ldm_f64_kernel"))) return rc;
+        }
     }
     const int ntiles = (npt + kTsThreads - 1) / kTsThreads;
+    const long long nb = (long long)npv * ntiles;
+    if (nb > 0x7fffffffLL) return set_error(RR_ERANGE, "ts_evaluate: grid too large");
+    // nsamples[0] as given by the host (0: unknown) selects the register-cached form
+    const bool cached = lc->uniform_nsamples >= 1 && lc->uniform_nsamples <= kTsGeoRegs;
+    const dim3 grid((unsigned)nb), block(kTsThreads);
     if (mixed) {
-        constexpr int PC = TsCfg<true>::kPbChunk;
-        const int nchunks = (npb + PC - 1) / PC;
-        const size_t lds = (size_t)PC * a.ng * 4 + 2 * PC * 4 + 8 + PC * 8;
-        const long long nb = (long long)npv * ntiles * nchunks;
-        if (nb > 0x7fffffffLL) return set_error(RR_ERANGE, "ts_evaluate: grid too large");
-        hipLaunchKernelGGL(ts_flux_kernel<true>, dim3((unsigned)nb), dim3(kTsThreads), lds, st, a, flux, ntiles, nchunks);
+        const int PC = ts_pb_chunk(true, cached);
+        const size_t lds = ((size_t)PC * a.ng + 3 * PC) * sizeof(float);
+        if (cached) hipLaunchKernelGGL((ts_flux_kernel<true, true>), grid, block, lds, st, a, flux, ntiles);
+        else hipLaunchKernelGGL((ts_flux_kernel<true, false>), grid, block, lds, st, a, flux, ntiles);
     } else {
-        constexpr int PC = TsCfg<false>::kPbChunk;
-        const int nchunks = (npb + PC - 1) / PC;
-        const size_t lds = (size_t)PC * a.ng * 8 + 2 * PC * 8 + PC * 8;
-        const long long nb = (long long)npv * ntiles * nchunks;
-        if (nb > 0x7fffffffLL) return set_error(RR_ERANGE, "ts_evaluate: grid too large");
-        hipLaunchKernelGGL(ts_flux_kernel<false>, dim3((unsigned)nb), dim3(kTsThreads), lds, st, a, flux, ntiles, nchunks);
+        const int PC = ts_pb_chunk(false, cached);
+        const size_t lds = ((size_t)PC * a.ng + 3 * PC) * sizeof(double);
+        if (cached) hipLaunchKernelGGL((ts_flux_kernel<false, true>), grid, block, lds, st, a, flux, ntiles);
+        else hipLaunchKernelGGL((ts_flux_kernel<false, false>), grid, block, lds, st, a, flux, ntiles);
     }
     return check_launch("ts_flux_kernel");
 }

@@ -208,13 +208,14 @@ def test_contiguous_light_curves_run_classification(gpu, oracle):
 
 
 def test_bjd_segments_many_passbands(gpu, oracle):
-    say('case: BJD 2-min cadence split into 7 contiguous passband light curves, nss=5, power-2')
+    say('case: BJD 2-min cadence split into 11 contiguous passband light curves, nss=5, power-2, k0 off-table')
     wl = workloads.rr_bjd(npv=32, npt=6000, seed=9)
     wl.nsamples = 5
-    npb = 7
+    npb = 11  # two limb-darkening groups in the setup kernel (8 + 3)
     lcids = (np.arange(wl.times.size) * npb) // wl.times.size
     rng = np.random.default_rng(9)
     wl.k = wl.k + 0.002 * np.sin(np.arange(npb) / 3.0)[None, :]
+    wl.k[0, 0] = 0.52  # outside klims: exact 2-D weights for every passband of pv 0
     wl.ldc = np.concatenate([wl.ldc] * npb, 1) + rng.uniform(-0.05, 0.05, (32, npb, 2))
     pbids = np.arange(npb)
     fo = oracle_flux(oracle, wl, 'power-2', lcids=lcids, pbids=pbids)

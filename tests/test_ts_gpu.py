@@ -88,3 +88,21 @@ def test_ts_equal_k_reduces_to_rr(gpu, oracle):
     fts = np.asarray(tm.evaluate(wl.k, None, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w), float)
     fo = oracle_ts(oracle, wl, ze)
     assert np.nanmax(np.abs(fts - fo)) <= FP64_TOL
+
+
+@pytest.mark.parametrize('prec,tol,nss', [('fp64', FP64_TOL, 2), ('mixed', MIXED_TOL, 2), ('fp64', FP64_TOL, 1)])
+def test_ts_wide_g_grid_and_two_samples(gpu, oracle, prec, tol, nss):
+    """ng = 200 puts the FP64 W2T (128 x 209 doubles) above the MFMA kernel's LDS
+    limit, so the FP64 contraction falls back to the vector ALUs; nss = 2 keeps
+    two samples' geometry in registers."""
+    say(f'case: TS ng=200, nss={nss}, {prec}')
+    wl, ze, mu = ts_case(oracle, npv=3, npt=500, npb=70)
+    fo = oracle.tsmodel_serial(wl.times, wl.k, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w, np.array([nss]),
+                               np.array([0.004]), wl.ldp, wl.istar, None, None, 0.005, 0.5, 200, None, ze)
+    tm = TSModel(TabulatedLDModel(wl.ldp, wl.istar), nzin=64, nzlimb=64, ng=200, precision=prec)
+    tm.set_data(wl.times, nsamples=nss, exptimes=0.004)
+    fg = np.asarray(tm.evaluate(wl.k, None, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w), float)
+    err = np.nanmax(np.abs(fg - fo))
+    say(f'  max |dF| = {err:.3e}')
+    assert (fo < 1).sum() > 10000
+    assert err <= tol
