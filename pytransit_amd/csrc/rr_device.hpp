@@ -279,12 +279,14 @@ __device__ __forceinline__ double ld_law(int law, double mu, const double* c, in
             const double u = a * b, v = a * (1.0 - b);
             return 1.0 - u * (1.0 - mu) - v * ((1.0 - mu) * (1.0 - mu));
         }
-        case RR_LD_NONLINEAR:
-            return 1.0 - c[0] * (1.0 - sqrt(mu)) - c[1] * (1.0 - mu) - c[2] * (1.0 - pow(mu, 1.5)) -
-                   c[3] * (1.0 - mu * mu);
+        case RR_LD_NONLINEAR: {
+            // mu^1.5 as mu * sqrt(mu): within 2 ulp of pow
+            const double sm = sqrt(mu);
+            return 1.0 - c[0] * (1.0 - sm) - c[1] * (1.0 - mu) - c[2] * (1.0 - mu * sm) - c[3] * (1.0 - mu * mu);
+        }
         case RR_LD_GENERAL: {
-            double s = 0.0;
-            for (int i = 0; i < nc; ++i) s += c[i] * (1.0 - pow(mu, (double)(i + 1)));
+            double s = 0.0, mp = mu;  // mp = mu^(i+1) by repeated products
+            for (int i = 0; i < nc; ++i, mp *= mu) s += c[i] * (1.0 - mp);
             return s;
         }
         case RR_LD_SQUARE_ROOT: return 1.0 - c[0] * (1.0 - mu) - c[1] * (1.0 - sqrt(mu));
@@ -330,6 +332,32 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+// 2*pi*trapz(z*I(mu), z) on mu = linspace(1, 0, 200) (rrmodel.py:151-152,220-227) by a
+// whole wavefront: each node's law value is computed once, segment m pairs node m
+// with node m-1 (from the next lower lane, or lane 63 of the previous round).
+__device__ __forceinline__ double ld_trapz_wave(int law, const double* c, int nc) {
+    const int lane = threadIdx.x & 63;
+    double part = 0.0, carry_z = 0.0, carry_y = 0.0;
+    for (int base = 0; base < 200; base += 64) {
+        const int m = base + lane;
+        double z = 0.0, y = 0.0;
+        if (m < 200) {
+            const double mu = linspace_at(1.0, 0.0, 200, m);
+            z = sqrt(1.0 - mu * mu);
+            y = z * ld_law(law, mu, c, nc);
+        }
+        double zl = __shfl_up(z, 1, 64), yl = __shfl_up(y, 1, 64);
+        if (lane == 0) {
+            zl = carry_z;
+            yl = carry_y;
+        }
+        if (m >= 1 && m < 200) part += (z - zl) * (y + yl) / 2.0;
+        carry_z = __shfl(z, 63, 64);
+        carry_y = __shfl(y, 63, 64);
+    }
+    return 2.0 * kPi * wave_sum(part);
 }
 
 }  // namespace rr

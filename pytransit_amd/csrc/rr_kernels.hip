@@ -211,6 +211,83 @@ __host__ __device__ inline int ld_group_size(int nz) {
     return g < kLdGroup ? (g < 1 ? 1 : g) : kLdGroup;
 }
 
+// Limb-darkening means of gn (<= G) passbands whose profiles are in s_l, lanes
+// over g; each table value read serves all G passbands.
+template <int G>
+__device__ __forceinline__ void ld_means(const SetupArgs& s, int pv, size_t j0, int gn, bool in_table, int ik,
+                                         double ak, double k0, const double* s_l) {
+    const int lane = threadIdx.x;
+    const int nz = s.nz, ng = s.ng;
+    const bool mixed = s.ldm32 != nullptr;
+    for (int ig = lane; ig < ng; ig += 64) {
+        double v[G];
+        if (in_table) {
+            double d0[G], d1[G];
+#pragma unroll
+            for (int q = 0; q < G; ++q) d0[q] = d1[q] = 0.0;
+            const size_t o0 = (size_t)ik * nz * ng + ig, o1 = o0 + (size_t)nz * ng;
+            // table values in batches of 8 rows, all loads issued before use
+            constexpr int kB = 8;
+            for (int iz0 = 0; iz0 < nz; iz0 += kB) {
+                double w0[kB], w1[kB];
+#pragma unroll
+                for (int u = 0; u < kB; ++u) {
+                    const size_t iz = (size_t)min(iz0 + u, nz - 1);
+                    if (mixed && s.W32) {
+                        w0[u] = (double)s.W32[o0 + iz * ng];
+                        w1[u] = (double)s.W32[o1 + iz * ng];
+                    } else {
+                        w0[u] = s.W[o0 + iz * ng];
+                        w1[u] = s.W[o1 + iz * ng];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kB; ++u) {
+                    if (iz0 + u < nz) {
+#pragma unroll
+                        for (int q = 0; q < G; ++q) {
+                            if (q < gn) {
+                                d0[q] += w0[u] * s_l[q * nz + iz0 + u];
+                                d1[q] += w1[u] * s_l[q * nz + iz0 + u];
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < G; ++q) v[q] = (1.0 - ak) * d0[q] + ak * d1[q];
+        } else {
+            // common.py:152-185: normalised annulus areas, recomputed in a second pass
+            const double g = linspace_at(0.0, 1.0 - 1e-7, ng, ig);
+            const double b = g * (1.0 + k0);
+            double a0 = cci_area(s.ze[0], k0, b), sum = a0;
+            for (int iz = 1; iz < nz; ++iz) {
+                const double a1 = cci_area(s.ze[iz], k0, b);
+                sum += a1 - a0;
+                a0 = a1;
+            }
+            double b0 = cci_area(s.ze[0], k0, b);
+#pragma unroll
+            for (int q = 0; q < G; ++q) v[q] = (b0 / sum) * s_l[(q < gn ? q : 0) * nz];
+            for (int iz = 1; iz < nz; ++iz) {
+                const double b1 = cci_area(s.ze[iz], k0, b);
+                const double wv = (b1 - b0) / sum;
+#pragma unroll
+                for (int q = 0; q < G; ++q)
+                    if (q < gn) v[q] += wv * s_l[q * nz + iz];
+                b0 = b1;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            if (q < gn) {
+                if (mixed) s.ldm32[(j0 + q) * ng + ig] = (float)v[q];
+                else s.ldm64[(j0 + q) * ng + ig] = v[q];
+            }
+        }
+    }
+}
+
 __device__ void setup_ld_role(const SetupArgs& s, int pv, int grp) {
     __shared__ double s_l[kLdLds];
     __shared__ double s_ist[kLdGroup];
@@ -228,7 +305,6 @@ __device__ void setup_ld_role(const SetupArgs& s, int pv, int grp) {
         ik = ik > s.nk - 2 ? s.nk - 2 : ik;
         ak = (k0 - s.kmin - ik * s.dk) / s.dk;
     }
-    const bool mixed = s.ldm32 != nullptr;
     const size_t j0 = (size_t)pv * s.npb + pb0;
     // profiles at the mu nodes and their disk integrals
     for (int q = 0; q < gn; ++q) {
@@ -241,11 +317,7 @@ __device__ void setup_ld_role(const SetupArgs& s, int pv, int grp) {
         } else {
             const double* c = s.ldc + j * s.nldc;
             for (int iz = lane; iz < nz; iz += 64) l[iz] = ld_law(s.law, s.mu[iz], c, s.nldc);
-            if (!ld_integral(s.law, c, ist)) {
-                double part = 0.0;
-                for (int m = 1 + lane; m < 200; m += 64) part += ld_trapz_segment(s.law, c, s.nldc, m);
-                ist = 2.0 * kPi * wave_sum(part);
-            }
+            if (!ld_integral(s.law, c, ist)) ist = ld_trapz_wave(s.law, c, s.nldc);
         }
         if (lane == 0) s_ist[q] = ist;
     }
@@ -254,62 +326,8 @@ __device__ void setup_ld_role(const SetupArgs& s, int pv, int grp) {
     if (lane == 0 && pb0 == 0) s.rec[pv].ldok = isnan(s_l[0]) ? 0 : 1;
     // limb-darkening means: lerp in k of two table rows, or exact 2-D weights
     // when k0 is outside [kmin, kmax]; model_full.py:47-55
-    for (int ig = lane; ig < ng; ig += 64) {
-        double v[kLdGroup];
-        if (in_table) {
-            double d0[kLdGroup], d1[kLdGroup];
-#pragma unroll
-            for (int q = 0; q < kLdGroup; ++q) d0[q] = d1[q] = 0.0;
-            const size_t o0 = (size_t)ik * nz * ng + ig, o1 = o0 + (size_t)nz * ng;
-            for (int iz = 0; iz < nz; ++iz) {
-                double w0, w1;
-                if (mixed && s.W32) {
-                    w0 = (double)s.W32[o0 + (size_t)iz * ng];
-                    w1 = (double)s.W32[o1 + (size_t)iz * ng];
-                } else {
-                    w0 = s.W[o0 + (size_t)iz * ng];
-                    w1 = s.W[o1 + (size_t)iz * ng];
-                }
-#pragma unroll
-                for (int q = 0; q < kLdGroup; ++q) {
-                    if (q < gn) {
-                        d0[q] += w0 * s_l[q * nz + iz];
-                        d1[q] += w1 * s_l[q * nz + iz];
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < kLdGroup; ++q) v[q] = (1.0 - ak) * d0[q] + ak * d1[q];
-        } else {
-            // common.py:152-185: normalised annulus areas, recomputed in a second pass
-            const double g = linspace_at(0.0, 1.0 - 1e-7, ng, ig);
-            const double b = g * (1.0 + k0);
-            double a0 = cci_area(s.ze[0], k0, b), sum = a0;
-            for (int iz = 1; iz < nz; ++iz) {
-                const double a1 = cci_area(s.ze[iz], k0, b);
-                sum += a1 - a0;
-                a0 = a1;
-            }
-            double b0 = cci_area(s.ze[0], k0, b);
-#pragma unroll
-            for (int q = 0; q < kLdGroup; ++q) v[q] = (b0 / sum) * s_l[(q < gn ? q : 0) * nz];
-            for (int iz = 1; iz < nz; ++iz) {
-                const double b1 = cci_area(s.ze[iz], k0, b);
-                const double wv = (b1 - b0) / sum;
-#pragma unroll
-                for (int q = 0; q < kLdGroup; ++q)
-                    if (q < gn) v[q] += wv * s_l[q * nz + iz];
-                b0 = b1;
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kLdGroup; ++q) {
-            if (q < gn) {
-                if (mixed) s.ldm32[(j0 + q) * ng + ig] = (float)v[q];
-                else s.ldm64[(j0 + q) * ng + ig] = v[q];
-            }
-        }
-    }
+    if (gn == 1) ld_means<1>(s, pv, j0, 1, in_table, ik, ak, k0, s_l);
+    else ld_means<kLdGroup>(s, pv, j0, gn, in_table, ik, ak, k0, s_l);
 }
 
 __device__ __forceinline__ void setup_body(const SetupArgs& s, int blk) {

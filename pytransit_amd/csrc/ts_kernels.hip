@@ -186,9 +186,12 @@ __global__ void __launch_bounds__(256) ts_weights_kernel(TsArgs s) {
     }
 }
 
+// Profile value of (pv, pb) at mu node iz: a table read, or the named law (the
+// template keeps the law code out of the table-read kernels' registers).
+template <bool kTab>
 __device__ __forceinline__ double ts_profile(const TsArgs& s, int pv, int pb, int iz) {
     const size_t j = (size_t)pv * s.npb + pb;
-    if (s.law == RR_LD_TABULATED) return s.ldp_in[j * s.nz + iz];
+    if (kTab) return s.ldp_in[j * s.nz + iz];
     return ld_law(s.law, s.mu[iz], s.ldc + j * s.nldc, s.nldc);
 }
 
@@ -200,6 +203,7 @@ __device__ __forceinline__ double ts_profile(const TsArgs& s, int pv, int pb, in
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 constexpr int kMfmaK = 128;   // k per register-resident A fragment
 
+template <bool kTab>
 __global__ void __launch_bounds__(256) ts_ldm_mfma_kernel(TsArgs s) {
     extern __shared__ float lds_b[];  // [nzp][ngp + 1]
     const int pv = blockIdx.x;
@@ -224,7 +228,7 @@ __global__ void __launch_bounds__(256) ts_ldm_mfma_kernel(TsArgs s) {
 #pragma unroll
                 for (int st = 0; st < kMfmaK / 4; ++st) {
                     const int iz = k0 + st * 4 + lk;
-                    afrag[st] = (row_ok && iz < s.nz) ? (float)ts_profile(s, pv, pb, iz) : 0.0f;
+                    afrag[st] = (row_ok && iz < s.nz) ? (float)ts_profile<kTab>(s, pv, pb, iz) : 0.0f;
                 }
 #pragma unroll
                 for (int st = 0; st < kMfmaK / 4; ++st) {
@@ -263,6 +267,7 @@ typedef double f64x4_t __attribute__((ext_vector_type(4)));
 constexpr int kMfma64Threads = 512;
 constexpr size_t kMfma64MaxLds = 128 * 1024;
 
+template <bool kTab>
 __global__ void __launch_bounds__(kMfma64Threads) ts_ldm_mfma64_kernel(TsArgs s) {
     extern __shared__ double lds_b64[];  // [nzp][ngp + 1]
     const int pv = blockIdx.x;
@@ -285,7 +290,7 @@ __global__ void __launch_bounds__(kMfma64Threads) ts_ldm_mfma64_kernel(TsArgs s)
             for (int c = 0; c < 8; ++c) acc[c] = (f64x4_t){0.0, 0.0, 0.0, 0.0};
             for (int k0 = 0; k0 < s.nzp; k0 += 4) {
                 const int iz = k0 + lk;
-                const double a = (row_ok && iz < s.nz) ? ts_profile(s, pv, pb, iz) : 0.0;
+                const double a = (row_ok && iz < s.nz) ? ts_profile<kTab>(s, pv, pb, iz) : 0.0;
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
                     const int ct = ct0 + c;
@@ -311,6 +316,7 @@ __global__ void __launch_bounds__(kMfma64Threads) ts_ldm_mfma64_kernel(TsArgs s)
 }
 
 // ---- FP64 contraction on the vector ALUs: one lane per (pv, pb, g) --------------------
+template <bool kTab>
 __global__ void __launch_bounds__(256) ts_ldm_f64_kernel(TsArgs s) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)s.npv * s.npb * s.ng) return;
@@ -319,7 +325,7 @@ __global__ void __launch_bounds__(256) ts_ldm_f64_kernel(TsArgs s) {
     const int pv = (int)(j / s.npb), pb = (int)(j - (long long)pv * s.npb);
     const double* w = s.w2_64 + (size_t)pv * s.nzp * s.ngp + ig;
     double d = 0.0;
-    for (int iz = 0; iz < s.nz; ++iz) d += w[(size_t)iz * s.ngp] * ts_profile(s, pv, pb, iz);
+    for (int iz = 0; iz < s.nz; ++iz) d += w[(size_t)iz * s.ngp] * ts_profile<kTab>(s, pv, pb, iz);
     s.ldm64[idx] = d;
 }
 
@@ -368,16 +374,36 @@ __device__ __forceinline__ TsGeo<Real> ts_geometry(const TsRec& r, double tc, do
     return o;
 }
 
-// With nsamples <= kTsGeoRegs the geometry of a time's samples stays in registers
-// and the passbands are the outer loop (one accumulator); otherwise the samples
-// are the outer loop over a chunk of PC accumulators and the geometry is
-// recomputed per chunk.
+// With nsamples <= kTsGeoRegs the geometry of a lane's samples stays in registers,
+// the passbands are the outer loop and a lane owns 16 bytes of consecutive times
+// (4 FP32 or 2 FP64), so a wave stores 1 KB of one passband row per instruction.  Otherwise a lane
+// owns one time, the samples are the outer loop over a chunk of PC accumulators
+// and the geometry is recomputed per chunk.
 constexpr int kTsGeoRegs = 2;
+// times per lane of the cached form: one 16-byte store per passband row
+template <typename Real> constexpr int ts_tpl() { return 16 / (int)sizeof(Real); }
+
+template <typename Real>
+__device__ __forceinline__ void ts_store(Real* o, const Real* v, int n, bool vec) {
+    if (vec && n == ts_tpl<Real>()) {
+        if constexpr (std::is_same<Real, float>::value) {
+            const f32x4_t w = {v[0], v[1], v[2], v[3]};
+            __builtin_nontemporal_store(w, (f32x4_t*)o);
+        } else {
+            typedef double f64x2_t __attribute__((ext_vector_type(2)));
+            const f64x2_t w = {v[0], v[1]};
+            __builtin_nontemporal_store(w, (f64x2_t*)o);
+        }
+    } else {
+        for (int j = 0; j < n; ++j) o[j] = v[j];
+    }
+}
 
 template <bool kMixed, bool kCached>
 __global__ void __launch_bounds__(kTsThreads) ts_flux_kernel(TsArgs s, void* flux, int ntiles) {
     using Real = typename std::conditional<kMixed, float, double>::type;
     constexpr int PC = ts_pb_chunk(kMixed, kCached);
+    constexpr int TPL = kCached ? ts_tpl<Real>() : 1;
     extern __shared__ __align__(16) unsigned char lds[];
     Real* s_ldm = (Real*)lds;                       // [PC][ng]
     Real* s_afac = s_ldm + (size_t)PC * s.ng;       // k^2 / k_mean^2
@@ -387,45 +413,62 @@ __global__ void __launch_bounds__(kTsThreads) ts_flux_kernel(TsArgs s, void* flu
 
     const int tile = blockIdx.x % ntiles;
     const int pv = blockIdx.x / ntiles;
-    const int t = tile * kTsThreads + threadIdx.x;
-    const bool tok = t < s.npt;
+    const int t0l = (tile * kTsThreads + (int)threadIdx.x) * TPL;  // first time of the lane
+    const int nt = max(0, min(TPL, s.npt - t0l));                   // its valid times
     const TsRec& r = s.rec[pv];
     Real* out = (Real*)flux + (size_t)pv * s.npb * s.npt;
+    // whole-vector stores need every row start 16-B aligned
+    const bool vec = TPL > 1 && (s.npt % TPL) == 0 && (((uintptr_t)out) % 16) == 0;
+    Real ones[TPL], v[TPL];
+#pragma unroll
+    for (int j = 0; j < TPL; ++j) ones[j] = Real(1);
     if (!r.good) {
-        if (tok)
-            for (int q = 0; q < s.npb; ++q) out[(size_t)q * s.npt + t] = Real(NAN);
+#pragma unroll
+        for (int j = 0; j < TPL; ++j) v[j] = Real(NAN);
+        if (nt > 0)
+            for (int q = 0; q < s.npb; ++q) ts_store(out + (size_t)q * s.npt + t0l, v, nt, vec);
         return;
     }
-    double tc = 0.0;
-    bool inw = false;
-    if (tok) {
+    double tc[TPL];
+    bool inw[TPL];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < TPL; ++j) {
+        tc[j] = 0.0;
+        inw[j] = false;
+        if (j < nt) {
 #pragma clang fp contract(off)
-        const double t0v = s.t0[pv];
-        const double epoch = floor((s.times[t] - t0v + 0.5 * r.p) * r.inv_p);
-        const double u = s.times[t] - t0v + 0.5 * r.p;
-        const double rem = u - epoch * r.p;
-        const double ep = epoch + ((rem < 0.0) ? -1.0 : ((rem >= r.p) ? 1.0 : 0.0));
-        tc = s.times[t] - (t0v + ep * r.p);
-        inw = (r.bt1 <= tc) && (tc <= r.bt4);
+            const double t = s.times[t0l + j];
+            const double t0v = s.t0[pv];
+            const double epoch = floor((t - t0v + 0.5 * r.p) * r.inv_p);
+            const double u = t - t0v + 0.5 * r.p;
+            const double rem = u - epoch * r.p;
+            const double ep = epoch + ((rem < 0.0) ? -1.0 : ((rem >= r.p) ? 1.0 : 0.0));
+            tc[j] = t - (t0v + ep * r.p);
+            inw[j] = (r.bt1 <= tc[j]) && (tc[j] <= r.bt4);
+            any = any || inw[j];
+        }
     }
     if (threadIdx.x == 0) s_any = 0;
     __syncthreads();
-    if (inw) s_any = 1;
+    if (any) s_any = 1;
     __syncthreads();
     if (!s_any) {
-        if (tok)
-            for (int q = 0; q < s.npb; ++q) out[(size_t)q * s.npt + t] = Real(1);
+        if (nt > 0)
+            for (int q = 0; q < s.npb; ++q) ts_store(out + (size_t)q * s.npt + t0l, ones, nt, vec);
         return;
     }
     const int ns = s.nsamples[0];
     const double ext = s.exptimes[0];
     const Real inv_ns = Real(1) / Real(ns);
     const double km = r.kmean;
-    TsGeo<Real> geo[kCached ? kTsGeoRegs : 1];
+    TsGeo<Real> geo[TPL][kCached ? kTsGeoRegs : 1];
     if (kCached) {
 #pragma unroll
-        for (int is = 0; is < kTsGeoRegs; ++is)
-            if (inw && is < ns) geo[is] = ts_geometry<Real>(r, tc, ext, ns, is, s.dg, s.ng);
+        for (int j = 0; j < TPL; ++j)
+#pragma unroll
+            for (int is = 0; is < kTsGeoRegs; ++is)
+                if (inw[j] && is < ns) geo[j][is] = ts_geometry<Real>(r, tc[j], ext, ns, is, s.dg, s.ng);
     }
 
     for (int pb0 = 0; pb0 < s.npb; pb0 += PC) {
@@ -443,8 +486,8 @@ __global__ void __launch_bounds__(kTsThreads) ts_flux_kernel(TsArgs s, void* flu
             s_iist[q] = (Real)(1.0 / s.istar[(size_t)pv * s.npb + pb0 + q]);
         }
         __syncthreads();
-        if (!tok) continue;
-        Real* o = out + (size_t)pb0 * s.npt + t;
+        if (nt == 0) continue;
+        Real* o = out + (size_t)pb0 * s.npt + t0l;
         // depth of passband q for one sample; the flux is 1 - mean(depth) / istar
         auto depth = [&](const TsGeo<Real>& g, int q) {
             const Real* l = s_ldm + (size_t)q * s.ng;
@@ -453,23 +496,27 @@ __global__ void __launch_bounds__(kTsThreads) ts_flux_kernel(TsArgs s, void* flu
             return ip * area;
         };
         if (kCached) {
-            if (!inw) {
-                for (int q = 0; q < npc; ++q) o[(size_t)q * s.npt] = Real(1);
-                continue;
-            }
-#pragma unroll 4
+#pragma unroll 2
             for (int q = 0; q < npc; ++q) {
-                Real acc = depth(geo[0], q);
-                if (ns > 1) acc += depth(geo[kCached ? kTsGeoRegs - 1 : 0], q);
-                o[(size_t)q * s.npt] = Real(1) - acc * (s_iist[q] * inv_ns);
+                const Real f = s_iist[q] * inv_ns;
+#pragma unroll
+                for (int j = 0; j < TPL; ++j) {
+                    Real acc = Real(0);
+                    if (inw[j]) {
+                        acc = depth(geo[j][0], q);
+                        if (ns > 1) acc += depth(geo[j][kTsGeoRegs - 1], q);
+                    }
+                    v[j] = inw[j] ? Real(1) - acc * f : Real(1);
+                }
+                ts_store(o + (size_t)q * s.npt, v, nt, vec);
             }
         } else {
             Real acc[PC];
 #pragma unroll
             for (int q = 0; q < PC; ++q) acc[q] = Real(0);
-            if (inw) {
+            if (inw[0]) {
                 for (int is = 0; is < ns; ++is) {
-                    const TsGeo<Real> g = ts_geometry<Real>(r, tc, ext, ns, is, s.dg, s.ng);
+                    const TsGeo<Real> g = ts_geometry<Real>(r, tc[0], ext, ns, is, s.dg, s.ng);
 #pragma unroll
                     for (int q = 0; q < PC; ++q)
                         if (q < npc) acc[q] += depth(g, q);
@@ -477,7 +524,7 @@ __global__ void __launch_bounds__(kTsThreads) ts_flux_kernel(TsArgs s, void* flu
             }
 #pragma unroll
             for (int q = 0; q < PC; ++q)
-                if (q < npc) o[(size_t)q * s.npt] = inw ? Real(1) - acc[q] * (s_iist[q] * inv_ns) : Real(1);
+                if (q < npc) o[(size_t)q * s.npt] = inw[0] ? Real(1) - acc[q] * (s_iist[q] * inv_ns) : Real(1);
         }
     }
 }
@@ -550,6 +597,7 @@ int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
     a.use_table = tab->weights != nullptr && tab->nk >= 2;
     a.kmin = tab->kmin; a.kmax_tab = tab->kmax; a.dk = tab->dk; a.dg = tab->dg;
     hipStream_t st = (hipStream_t)stream;
+    const bool tab_ld = ld->law == RR_LD_TABULATED;
 
     hipLaunchKernelGGL(ts_setup_kernel, dim3((npv + kTsGroup - 1) / kTsGroup), dim3(64), 0, st, a);
     int rc = check_launch("ts_setup_kernel");
@@ -562,24 +610,29 @@ int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
     if (mixed) {
         const size_t lds = (size_t)a.nzp * (a.ngp + 1) * sizeof(float);
         if (lds > 160 * 1024) return set_error(RR_ERANGE, "ts_evaluate: nz*ng too large for the MFMA kernel");
-        hipLaunchKernelGGL(ts_ldm_mfma_kernel, dim3(npv), dim3(256), lds, st, a);
+        if (tab_ld) hipLaunchKernelGGL(ts_ldm_mfma_kernel<true>, dim3(npv), dim3(256), lds, st, a);
+        else hipLaunchKernelGGL(ts_ldm_mfma_kernel<false>, dim3(npv), dim3(256), lds, st, a);
         if ((rc = check_launch("ts_ldm_mfma_kernel"))) return rc;
     } else {
         const size_t lds = (size_t)a.nzp * (a.ngp + 1) * sizeof(double);
         if (lds <= kMfma64MaxLds) {
-            hipLaunchKernelGGL(ts_ldm_mfma64_kernel, dim3(npv), dim3(kMfma64Threads), lds, st, a);
+            if (tab_ld) hipLaunchKernelGGL(ts_ldm_mfma64_kernel<true>, dim3(npv), dim3(kMfma64Threads), lds, st, a);
+            else hipLaunchKernelGGL(ts_ldm_mfma64_kernel<false>, dim3(npv), dim3(kMfma64Threads), lds, st, a);
             if ((rc = check_launch("ts_ldm_mfma64_kernel"))) return rc;
         } else {
             const long long n = (long long)npv * npb * a.ng;
-            hipLaunchKernelGGL(ts_ldm_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+            const dim3 g((unsigned)((n + 255) / 256));
+            if (tab_ld) hipLaunchKernelGGL(ts_ldm_f64_kernel<true>, g, dim3(256), 0, st, a);
+            else hipLaunchKernelGGL(ts_ldm_f64_kernel<false>, g, dim3(256), 0, st, a);
             if ((rc = check_launch("ts_ldm_f64_kernel"))) return rc;
         }
     }
-    const int ntiles = (npt + kTsThreads - 1) / kTsThreads;
-    const long long nb = (long long)npv * ntiles;
-    if (nb > 0x7fffffffLL) return set_error(RR_ERANGE, "ts_evaluate: grid too large");
     // nsamples[0] as given by the host (0: unknown) selects the register-cached form
     const bool cached = lc->uniform_nsamples >= 1 && lc->uniform_nsamples <= kTsGeoRegs;
+    const int per_tile = kTsThreads * (cached ? (mixed ? ts_tpl<float>() : ts_tpl<double>()) : 1);
+    const int ntiles = (npt + per_tile - 1) / per_tile;
+    const long long nb = (long long)npv * ntiles;
+    if (nb > 0x7fffffffLL) return set_error(RR_ERANGE, "ts_evaluate: grid too large");
     const dim3 grid((unsigned)nb), block(kTsThreads);
     if (mixed) {
         const int PC = ts_pb_chunk(true, cached);
