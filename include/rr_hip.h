@@ -207,6 +207,17 @@ int rr_lnlike(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_
               const rr_ld* ld, const double* obs, const double* sigma, double* lnl, double* grad,
               void* stream);
 
+/* Mandel & Agol (2002) quadratic limb-darkening model: flux [npv, npt] (FP64) for
+ * ldc [npv, 2*npb] = (u1, u2) per passband, k [npv, 1|npb], t0 [npv, nep], with the
+ * Taylor orbit and contact window of the RoadRunner path widened by 0.025 d and
+ * per-light-curve supersampling.  A NaN radius ratio or semi-major axis gives an
+ * inf row, NaN coefficients inf points.  Replaces quadratic_model_v
+ * (pytransit/models/numba/ma_quadratic_nb.py:532-595) and eval_quad_z_s (:277-389)
+ * as called by QuadraticModel.evaluate (pytransit/models/ma_quadratic.py:74-140). */
+size_t ma_workspace_bytes(int npv);
+int ma_evaluate(const rr_lcdata* lc, const rr_params* pv, const double* ldc, void* workspace,
+                size_t workspace_bytes, double* flux, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

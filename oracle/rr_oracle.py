@@ -823,3 +823,43 @@ def eval_quad_z(z: float, k: float, u) -> float:
             ed = 1.0 / 2.0 * inv_pi * (kap1 + k2 * (k2 + 2.0 * z2) * kap0
                                        - (1.0 + 5.0 * k2 + z2) / 4.0 * math.sqrt((1.0 - x1) * (x2 - 1.0)))
     return 1.0 - ((1.0 - u[0] - 2.0 * u[1]) * le + (u[0] + 2.0 * u[1]) * ld + u[1] * ed) / omega
+
+
+def quadratic_model(times, k, t0, p, a, i, e, w, ldc, lcids, pbids, epids, nsamples, exptimes):
+    """Mandel-Agol light curves of a parameter population; quadratic_model_v,
+    ma_quadratic_nb.py:532-595 (the non-interpolated branch).  k [npv, 1|npb],
+    t0 [npv, nep], ldc [npv, 2*npb]; flux [npv, npt].  A NaN k or a gives an inf
+    row, NaN coefficients inf points; the window is the contact times +- 0.025 d."""
+    times = np.asarray(times, float)
+    k, t0, ldc = np.atleast_2d(k), np.atleast_2d(t0), np.atleast_2d(ldc)
+    npv, npt = k.shape[0], times.size
+    flux = np.zeros((npv, npt))
+    for ipv in range(npv):
+        if np.isnan(k[ipv]).any() or math.isnan(a[ipv]):
+            flux[ipv] = np.inf
+            continue
+        c = taylor_coeffs(p[ipv], a[ipv], i[ipv], e[ipv], w[ipv])
+        bt1, bt4 = bounding_box(k[ipv, 0], c)
+        bt1 -= 0.025
+        bt4 += 0.025
+        for j in range(npt):
+            ilc = lcids[j]
+            iep = epids[ilc]
+            epoch = math.floor((times[j] - t0[ipv, iep] + 0.5 * p[ipv]) / p[ipv])
+            tc = times[j] - (t0[ipv, iep] + epoch * p[ipv])
+            if not (bt1 <= tc <= bt4):
+                flux[ipv, j] = 1.0
+                continue
+            ipb = pbids[ilc]
+            kk = k[ipv, 0] if k.shape[1] == 1 else k[ipv, ipb]
+            u = ldc[ipv, 2 * ipb:2 * (ipb + 1)]
+            if np.isnan(u).any():
+                flux[ipv, j] = np.inf
+                continue
+            acc = 0.0
+            ns = nsamples[ilc]
+            for s in range(1, ns + 1):
+                z = sep(tc + exptimes[ilc] * ((s - 0.5) / ns - 0.5), c)
+                acc += 1.0 if z > 1.0 + kk else eval_quad_z(z, kk, u)
+            flux[ipv, j] = acc / ns
+    return flux

@@ -2,12 +2,16 @@
 
 Public API mirrors ``pytransit`` (``pytransit/__init__.py:64-66``):
 ``RoadRunnerModel`` (alias ``RRModel``), ``TransmissionSpectroscopyModel`` (alias
-``TSModel``) and the ``LDModel`` plug-in protocol.
+``TSModel``), ``QuadraticModel`` (Mandel & Agol) and the ``LDModel`` plug-in
+protocol; ``RoadRunnerLPFModel`` takes the layouts of PyTransit's LPFs.
 """
 from .ldmodel import LDModel, TabulatedLDModel
+from .lpf import RoadRunnerLPFModel
+from .qmodel import QuadraticModel
 from .rrmodel import RoadRunnerModel, RRModel
 from .tsmodel import TransmissionSpectroscopyModel, TSModel
 
 __version__ = '0.1.0'
 
-__all__ = ['RoadRunnerModel', 'RRModel', 'TransmissionSpectroscopyModel', 'TSModel', 'LDModel', 'TabulatedLDModel']
+__all__ = ['RoadRunnerModel', 'RRModel', 'TransmissionSpectroscopyModel', 'TSModel', 'QuadraticModel',
+           'RoadRunnerLPFModel', 'LDModel', 'TabulatedLDModel']

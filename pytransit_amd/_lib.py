@@ -82,6 +82,8 @@ _SIGNATURES = {
     'ts_evaluate': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd),
                             c_void_p, c_size_t, c_void_p, c_void_p]),
     'rr_max_batch': (c_int, []),
+    'ma_workspace_bytes': (c_size_t, [c_int]),
+    'ma_evaluate': (c_int, [POINTER(RRLcData), POINTER(RRParams), c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
     'rr_evaluate_batch': (c_int, [c_int, POINTER(RRTable), c_int, POINTER(RRRequest), c_void_p]),
 }
 

@@ -327,6 +327,33 @@ __device__ __forceinline__ double ld_trapz_segment(int law, const double* c, int
     return (z1 - z0) * (z1 * ld_law(law, mu1, c, nc) + z0 * ld_law(law, mu0, c, nc)) / 2.0;
 }
 
+// ---- Orbit for 8 parameter vectors per 64-lane block: lane j (0..6) of a group of 8
+// solves Kepler at t = (j-3)*dt, lane 0 forms the stencils, lanes 0 and 1 find
+// the first and fourth contacts.  Writes coefficients[10] and the window.
+constexpr int kOrbitGroup = 8;
+
+__device__ __forceinline__ void orbit_group(int pv, bool valid, double p, double a, double inc, double e, double w,
+                                            double kw, double* out_c, double* out_win) {
+    __shared__ double s_x[8][8], s_y[8][8], s_c[8][10], s_w[8][2];
+    const int sub = threadIdx.x / kOrbitGroup, j = threadIdx.x % kOrbitGroup;
+    if (valid && j < 7) {
+        double x, y;
+        kepler_xy((j - 3) * 2e-2, p, a, cos(inc), e, w, x, y);
+        s_x[sub][j] = x;
+        s_y[sub][j] = y;
+    }
+    __syncthreads();
+    if (valid && j == 0) taylor_from_stencil(s_x[sub], s_y[sub], s_c[sub]);
+    __syncthreads();
+    if (valid && j < 2) s_w[sub][j] = find_contact_point(kw, j == 0 ? 1 : 4, s_c[sub]);
+    __syncthreads();
+    if (valid) {
+        out_c[j] = s_c[sub][j];
+        if (j < 2) out_c[8 + j] = s_c[sub][8 + j];
+        if (j < 2) out_win[j] = s_w[sub][j];
+    }
+}
+
 // Sum over a wavefront (64 lanes) by butterfly shuffles; every lane gets the total.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

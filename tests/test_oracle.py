@@ -374,3 +374,22 @@ def test_c_restatement_matches_numpy_oracle(oracle):
     fc = cm.evaluate(wl.k, wl.ldc, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w, nthreads=2)
     assert np.abs(fc - fo).max() < 1e-12
     assert (fo < 1).any()
+
+
+def test_mandel_agol_light_curve_matches_roadrunner(oracle):
+    """The oracle's full Mandel-Agol light curve (quadratic_model) against RoadRunner
+    with quadratic LD on the same orbit: equal to RR's discretisation error, and
+    exactly 1 outside transit; a NaN radius ratio gives an inf row."""
+    t = np.linspace(-0.12, 0.12, 301)
+    k, t0, p, a, i = 0.1, 0.0, 3.0, 10.0, 1.54
+    u = [0.3, 0.2]
+    one = np.zeros(t.size, int)
+    fma = oracle.quadratic_model(t, [[k]], [[t0]], [p], [a], [i], [0.0], [0.0], [u], one, [0], [0], [1], [0.0])[0]
+    om = oracle.OracleRR('quadratic')
+    om.set_data(t)
+    frr = om.evaluate(np.array([[k]]), np.array([[u]]), np.array([[t0]]), np.array([p]), np.array([a]),
+                      np.array([i]))[0]
+    assert np.abs(fma - frr).max() < 2e-5
+    assert (fma < 1).sum() > 50 and (fma == 1.0).sum() > 50
+    f2 = oracle.quadratic_model(t, [[np.nan]], [[t0]], [p], [a], [i], [0.0], [0.0], [u], one, [0], [0], [1], [0.0])
+    assert np.isinf(f2).all()
