@@ -12,9 +12,10 @@ namespace rr {
 // Per-parameter-vector state written by the setup kernel and read by the flux
 // kernels: Taylor coefficients (x then y, divided by n!), the contact window
 // (first/fourth contact, before the per-light-curve margin), period, 1/period,
-// the radius ratio the window and the k-table lerp used, and two validity flags
-// (orbit parameters valid; limb-darkening profile not NaN), written by the two
-// block roles of the setup kernel.
+// the radius ratio the window and the k-table lerp used, two validity flags
+// (orbit parameters valid; limb-darkening profile not NaN), and the per-pv values
+// of light curve 0 (its zero epoch, radius ratio and stellar disk integral), so a
+// one-light-curve flux tile needs no loads beyond this record.
 struct alignas(16) PvRec {
     double c[10];
     double bt1, bt4;
@@ -22,8 +23,10 @@ struct alignas(16) PvRec {
     double k0;
     int32_t good;
     int32_t ldok;
+    double t0_lc0, k_lc0, ist_lc0;
+    double pad_[5];
 };
-static_assert(sizeof(PvRec) == 128, "PvRec must be 128 bytes");
+static_assert(sizeof(PvRec) == 192, "PvRec must be 192 bytes");
 
 int set_error(int code, const char* fmt, ...);
 int check_launch(const char* what);

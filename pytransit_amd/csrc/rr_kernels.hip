@@ -148,9 +148,14 @@ struct SetupArgs {
     const double* ldc; const double* ldp_in; const double* istar_in;
     const double* W; const float* W32; const double* ze; const double* mu;
     PvRec* rec; double* istar; double* ldm64; float* ldm32;
-    int npv, npb, kcols, nk, ng, nz, n_orbit_blocks;
+    const double* t0; const int32_t* pbids; const int32_t* epids;   // light curve 0's context
+    int npv, npb, kcols, t0cols, nk, ng, nz, n_orbit_blocks;
     double kmin, kmax, dk;
 };
+
+// Passband and epoch index of light curve 0, clamped as the flux kernel clamps.
+__device__ __forceinline__ int lc0_pb(const int32_t* pbids, int npb) { return min(max(pbids[0], 0), npb - 1); }
+__device__ __forceinline__ int lc0_ep(const int32_t* epids, int nep) { return min(max(epids[0], 0), nep - 1); }
 
 __device__ void setup_orbit_role(const SetupArgs& s, int blk) {
     const int pv = blk * kOrbitGroup + threadIdx.x / kOrbitGroup;
@@ -167,6 +172,9 @@ __device__ void setup_orbit_role(const SetupArgs& s, int blk) {
         rec->inv_p = 1.0 / s.p[q];
         rec->k0 = k0;
         rec->good = valid ? 1 : 0;
+        const int ipb = lc0_pb(s.pbids, s.npb);
+        rec->t0_lc0 = s.t0[(size_t)q * s.t0cols + lc0_ep(s.epids, s.t0cols)];
+        rec->k_lc0 = s.k[(size_t)q * s.kcols + (s.kcols == 1 ? 0 : ipb)];
     }
 }
 
@@ -259,6 +267,56 @@ __device__ __forceinline__ void ld_means(const SetupArgs& s, int pv, size_t j0, 
     }
 }
 
+// One passband inside the table: each lane takes two g values per round, so twice
+// the table loads are in flight per batch (the per-block latency bounds setup).
+__device__ __forceinline__ void ld_means_single(const SetupArgs& s, size_t j, int ik, double ak, const double* s_l) {
+    const int lane = threadIdx.x;
+    const int nz = s.nz, ng = s.ng;
+    const bool mixed = s.ldm32 != nullptr;
+    constexpr int kB = 8;
+    for (int ig0 = lane; ig0 < ng; ig0 += 128) {
+        const int ig1 = min(ig0 + 64, ng - 1);
+        const size_t a0 = (size_t)ik * nz * ng, a1 = a0 + (size_t)nz * ng;
+        double d00 = 0.0, d01 = 0.0, d10 = 0.0, d11 = 0.0;  // [row][g]
+        for (int iz0 = 0; iz0 < nz; iz0 += kB) {
+            double w00[kB], w01[kB], w10[kB], w11[kB];
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const size_t r = (size_t)min(iz0 + u, nz - 1) * ng;
+                if (mixed && s.W32) {
+                    w00[u] = (double)s.W32[a0 + r + ig0];
+                    w01[u] = (double)s.W32[a0 + r + ig1];
+                    w10[u] = (double)s.W32[a1 + r + ig0];
+                    w11[u] = (double)s.W32[a1 + r + ig1];
+                } else {
+                    w00[u] = s.W[a0 + r + ig0];
+                    w01[u] = s.W[a0 + r + ig1];
+                    w10[u] = s.W[a1 + r + ig0];
+                    w11[u] = s.W[a1 + r + ig1];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                if (iz0 + u < nz) {
+                    const double l = s_l[iz0 + u];
+                    d00 += w00[u] * l;
+                    d01 += w01[u] * l;
+                    d10 += w10[u] * l;
+                    d11 += w11[u] * l;
+                }
+            }
+        }
+        const double v0 = (1.0 - ak) * d00 + ak * d10, v1 = (1.0 - ak) * d01 + ak * d11;
+        if (mixed) {
+            s.ldm32[j * ng + ig0] = (float)v0;
+            if (ig0 + 64 < ng) s.ldm32[j * ng + ig1] = (float)v1;
+        } else {
+            s.ldm64[j * ng + ig0] = v0;
+            if (ig0 + 64 < ng) s.ldm64[j * ng + ig1] = v1;
+        }
+    }
+}
+
 __device__ void setup_ld_role(const SetupArgs& s, int pv, int grp) {
     __shared__ double s_l[kLdLds];
     __shared__ double s_ist[kLdGroup];
@@ -295,9 +353,14 @@ __device__ void setup_ld_role(const SetupArgs& s, int pv, int grp) {
     __syncthreads();
     if (lane < gn) s.istar[j0 + lane] = s_ist[lane];
     if (lane == 0 && pb0 == 0) s.rec[pv].ldok = isnan(s_l[0]) ? 0 : 1;
+    {
+        const int q0 = lc0_pb(s.pbids, s.npb) - pb0;
+        if (lane == 0 && q0 >= 0 && q0 < gn) s.rec[pv].ist_lc0 = s_ist[q0];
+    }
     // limb-darkening means: lerp in k of two table rows, or exact 2-D weights
     // when k0 is outside [kmin, kmax]; model_full.py:47-55
-    if (gn == 1) ld_means<1>(s, pv, j0, 1, in_table, ik, ak, k0, s_l);
+    if (gn == 1 && in_table) ld_means_single(s, j0, ik, ak, s_l);
+    else if (gn == 1) ld_means<1>(s, pv, j0, 1, in_table, ik, ak, k0, s_l);
     else ld_means<kLdGroup>(s, pv, j0, gn, in_table, ik, ak, k0, s_l);
 }
 
@@ -449,6 +512,21 @@ __device__ __forceinline__ PointCtx point_ctx(const FluxArgs& f, const PvRec& r,
     return c;
 }
 
+// Light curve 0's context from the per-pv record (one light curve: no dependent loads).
+__device__ __forceinline__ PointCtx point_ctx0(const FluxArgs& f, const PvRec& r) {
+    PointCtx c;
+    c.ipb = min(max(f.pbids[0], 0), f.npb - 1);
+    c.ns = f.nsamples[0];
+    c.ext = f.exptimes[0];
+    c.kk = r.k_lc0;
+    c.ist = r.ist_lc0;
+    c.t0v = r.t0_lc0;
+    const double margin = 0.003 + c.ext;
+    c.lo = r.bt1 - margin;
+    c.hi = r.bt4 + margin;
+    return c;
+}
+
 // Contribution of sample `is` (0-based) of a point at phase tc: FP64 gives the
 // reference's (istar - I*A)/istar, FP32 gives the depth I*A (the flux is
 // 1 - mean(depth)/istar).
@@ -519,7 +597,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     if (kFast) {
         // light curves are contiguous and each is sorted in time: a run whose first
         // and last points share a light curve lies in that light curve, in order
-        const PointCtx c1 = point_ctx(f, r, pv, 0);
+        const PointCtx c1 = f.nlc == 1 ? point_ctx0(f, r) : point_ctx(f, r, pv, 0);
 #pragma unroll
         for (int m = 0; m < kRuns; ++m) {
             bool all_out = true, one_lc = true, one_ep = false;
@@ -602,7 +680,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     if (f.uniform_ns > 0 && f.nlc == 1 && f.row_lds) {
         // one light curve: the per-point context is uniform
         const int ns = f.uniform_ns;
-        const PointCtx c = point_ctx(f, r, pv, 0);
+        const PointCtx c = point_ctx0(f, r);
         const float inv_ns = 1.0f / (float)ns;
         const float kf = (float)c.kk, extf = (float)c.ext;
         float cf[10];
@@ -859,9 +937,11 @@ static int prepare_setup(const char* who, int precision, const rr_table* tab, co
                              kMinCoeffs[ld->law], ld->nldc);
         if (ld->nldc > 0 && !ld->ldc) return set_error(RR_EINVAL, "%s: null ldc", who);
     }
-    if (!pv->k || !pv->p || !pv->a || !pv->inc || !pv->e || !pv->w)
+    if (!pv->k || !pv->p || !pv->a || !pv->inc || !pv->e || !pv->w || !pv->t0)
         return set_error(RR_EINVAL, "%s: null parameter array", who);
+    if (!lc->pbids || !lc->epids) return set_error(RR_EINVAL, "%s: null light-curve array", who);
     WsView v = carve(workspace, npv, lc->npt, npb, tab->ng);
+    sa.t0 = pv->t0; sa.pbids = lc->pbids; sa.epids = lc->epids; sa.t0cols = pv->t0cols;
     sa.k = pv->k; sa.p = pv->p; sa.a = pv->a; sa.inc = pv->inc; sa.e = pv->e; sa.w = pv->w;
     sa.law = ld->law; sa.nldc = ld->nldc; sa.ldc = ld->ldc; sa.ldp_in = ld->ldp; sa.istar_in = ld->istar;
     sa.W = tab->weights; sa.W32 = tab->weights32; sa.ze = tab->ze; sa.mu = tab->mu;
