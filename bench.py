@@ -82,10 +82,11 @@ def copy_probe(torch, dev):
 
 def flux_kernel_bytes(npv, npt, ng, out_bytes):
     """Algorithmic HBM bytes of one flux launch: the flux write, one read of the
-    times, and per pv the setup record (128 B), its limb-darkening-mean row
-    (ng values) and k, t0, istar (8 B each)."""
+    times, and per pv its 192-byte setup record (orbit coefficients, window and
+    light curve 0's zero epoch, radius ratio and disk integral) and its
+    limb-darkening-mean row (ng values)."""
     row = ng * (4 if out_bytes == 4 else 8)
-    return npv * npt * out_bytes + npt * 8 + npv * (128 + row + 24)
+    return npv * npt * out_bytes + npt * 8 + npv * (192 + row)
 
 
 def time_path(tm, args_dev, steps, warmup):
