@@ -492,6 +492,23 @@ __device__ __forceinline__ void wave_fill(T* out, int i0, int n, T v, int wl) {
     for (int i = head + nv * kVec + wl; i < n; i += 64) base[i] = v;
 }
 
+// The n (<= kRun) times of a run starting at i0 (a multiple of kRun) into
+// registers: four 16-byte loads for a full run of a 16-byte aligned array.
+__device__ __forceinline__ void load_run(const double* times, int i0, int n, double* tv) {
+    if (n == kRun && (((uintptr_t)times) & 15) == 0) {
+        const f64x2* v = (const f64x2*)(times + i0);
+#pragma unroll
+        for (int q = 0; q < kRun / 2; ++q) {
+            const f64x2 x = v[q];
+            tv[2 * q] = x[0];
+            tv[2 * q + 1] = x[1];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kRun; ++j) tv[j] = j < n ? times[i0 + j] : 0.0;
+    }
+}
+
 // Per-light-curve quantities of one point (model_full.py:84-86, 93-98).
 struct PointCtx {
     int ipb, ns;
@@ -629,6 +646,11 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
             }
             if (all_out && nrun[m] > 0)
                 for (int j = 0; j < nrun[m]; ++j) out[its[m] + j] = Real(1);
+            // the run's times, all loads issued before the stores below (which
+            // the compiler may not move loads across)
+            double tv[kRun];
+            if (!all_out) load_run(f.times, its[m], nrun[m], tv);
+#pragma unroll
             for (int j = 0; j < kRun; ++j) {
                 bool inw = false;
                 const int it = its[m] + j;
@@ -636,11 +658,11 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                     double tc;
                     PointCtx cc = c;
                     if (one_ep) {
-                        tc = f.times[it] - base;  // model_full.py:89 with the run's epoch
+                        tc = tv[j] - base;  // model_full.py:89 with the run's epoch
                     } else {
                         if (!one_lc) cc = point_ctx(f, r, pv, min(max(f.lcids[it], 0), f.nlc - 1));
                         double ep;
-                        tc = phase_of(f.times[it], cc.t0v, p, inv_p, ep);
+                        tc = phase_of(tv[j], cc.t0v, p, inv_p, ep);
                     }
                     inw = (cc.lo <= tc) && (tc <= cc.hi);
                     if (!inw) out[it] = Real(1);
@@ -651,6 +673,9 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     } else {
 #pragma unroll
         for (int m = 0; m < kRuns; ++m) {
+            double tv[kRun];
+            load_run(f.times, its[m], nrun[m], tv);
+#pragma unroll
             for (int j = 0; j < kRun; ++j) {
                 bool inw = false;
                 const int it = its[m] + j;
@@ -658,7 +683,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                     const int ilc = f.nlc == 1 ? 0 : min(max(f.lcids[it], 0), f.nlc - 1);
                     const PointCtx c = point_ctx(f, r, pv, ilc);
                     double ep;
-                    const double tc = phase_of(f.times[it], c.t0v, p, inv_p, ep);
+                    const double tc = phase_of(tv[j], c.t0v, p, inv_p, ep);
                     inw = (c.lo <= tc) && (tc <= c.hi);
                     if (!inw) out[it] = Real(1);
                 }
