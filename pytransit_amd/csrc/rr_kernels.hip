@@ -570,6 +570,29 @@ __device__ __forceinline__ float sample_f32(const float* cf, float kf, float inv
     return ip * ap;
 }
 
+// The depth of one sample of the one-light-curve item loop: as sample_f32 for a
+// planet smaller than the star (kf < 1), where z < 1 + k gives g < 1 (no g > 1
+// branch) and the lens sort of (1, k, z) is three selects.  Returns I * A.
+__device__ __forceinline__ float sample_small_f32(const float* cf, float kf, float inv_dg, float tz, const float* lrow,
+                                                  int ng) {
+    const float zlim = 1.0f + kf;
+    const float z = taylor_z<float>(tz, cf);
+    if (!(z < zlim)) return 0.0f;
+    // limb-darkening mean at g = z / (1 + k) in [0, 1): common.py:225-233
+    const float x = z * (1.0f / zlim) * inv_dg;
+    const int i = min((int)x, ng - 2);
+    const float a = x - (float)i;
+    const float l0 = lrow[i], l1 = lrow[i + 1];
+    const float ip = fmaf(a, l1 - l0, l0);
+    if (z <= 1.0f - kf) return ip * (kPiF * (kf * kf));  // planet fully inside the disk
+    // lens: the kite of (1, k, z) sorted descending (common.py:5-33,52-73)
+    const float xs = d_max(1.0f, z), ys = d_max(kf, d_min(1.0f, z)), zs = d_min(kf, z);
+    const float sq = d_sqrt((xs + (ys + zs)) * (zs - (xs - ys)) * (zs + (xs - ys)) * (xs + (ys - zs)));
+    const float k0 = atan2_ypos_f(sq, (kf - 1.0f) * (kf + 1.0f) + z * z);
+    const float k1 = atan2_ypos_f(sq, (1.0f - kf) * (1.0f + kf) + z * z);
+    return ip * (k1 + kf * kf * k0 - 0.5f * sq);
+}
+
 template <bool kMixed, bool kFast>
 __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     using Real = typename std::conditional<kMixed, float, double>::type;
@@ -712,6 +735,10 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
 #pragma unroll
         for (int k = 0; k < 10; ++k) cf[k] = (float)r.c[k];
         float* s_tcf = (float*)s_tc;
+        // exposure offsets of the samples, exptime ((is + 1/2) / ns - 1/2) (model_full.py:94)
+        float* s_off = s_tcf + kMaxChunk;
+        if (kMixed && wl < ns) s_off[wl] = extf * (((float)wl + 0.5f) * inv_ns - 0.5f);
+        const bool small = kMixed && kf < 1.0f && ns <= kThreads;
         const int cp = f.chunk_pts;
         for (int c0 = 0; c0 < n; c0 += cp) {
             const int cn = min(cp, n - c0);
@@ -722,14 +749,28 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                 else s_tc[wl] = tc;
             }
             __syncthreads();
+            // item i = (point qq, sample is), i = qq * ns + is, advanced by 64 per round
+            int qq = (int)(((float)wl + 0.5f) * inv_ns);  // exact i / ns for i, ns <= 1024
+            int is = wl - qq * ns;
+            const int dq = kThreads / ns, ds = kThreads - dq * ns;
 #pragma unroll 1
             for (int i = wl; i < cn * ns; i += kThreads) {
-                const int qq = (int)(((float)i + 0.5f) * inv_ns);  // exact i / ns for i, ns <= 1024
-                const int is = i - qq * ns;
-                if (kMixed)
-                    s_dep[i] = sample_f32(cf, kf, inv_dg, s_tcf[qq], extf, inv_ns, (const float*)s_row, f.ng, is);
-                else
+                if (kMixed) {
+                    if (small) {
+                        const float tz = s_tcf[qq] + s_off[is];
+                        s_dep[i] = sample_small_f32(cf, kf, inv_dg, tz, (const float*)s_row, f.ng);
+                    } else {
+                        s_dep[i] = sample_f32(cf, kf, inv_dg, s_tcf[qq], extf, inv_ns, (const float*)s_row, f.ng, is);
+                    }
+                } else {
                     s_dep[i] = sample_f64(r.c, f.dg, f.ng, c, (const double*)s_row, s_tc[qq], is);
+                }
+                qq += dq;
+                is += ds;
+                if (is >= ns) {
+                    is -= ns;
+                    ++qq;
+                }
             }
             __syncthreads();
             if (wl < cn) {
