@@ -223,18 +223,22 @@ __device__ __forceinline__ void ld_means(const SetupArgs& s, int pv, size_t j0, 
 #pragma unroll
                 for (int u = 0; u < kB; ++u) {
                     if (iz0 + u < nz) {
+                        if constexpr (G == 1) {
+                            d0[0] += w0[u] * s_l[iz0 + u];
+                            d1[0] += w1[u] * s_l[iz0 + u];
+                        } else {
+                            // several passbands: lerp the table value once, then one
+                            // product per passband (the same sum within rounding)
+                            const double wk = (1.0 - ak) * w0[u] + ak * w1[u];
 #pragma unroll
-                        for (int q = 0; q < G; ++q) {
-                            if (q < gn) {
-                                d0[q] += w0[u] * s_l[q * nz + iz0 + u];
-                                d1[q] += w1[u] * s_l[q * nz + iz0 + u];
-                            }
+                            for (int q = 0; q < G; ++q)
+                                if (q < gn) d0[q] += wk * s_l[q * nz + iz0 + u];
                         }
                     }
                 }
             }
 #pragma unroll
-            for (int q = 0; q < G; ++q) v[q] = (1.0 - ak) * d0[q] + ak * d1[q];
+            for (int q = 0; q < G; ++q) v[q] = G == 1 ? (1.0 - ak) * d0[q] + ak * d1[q] : d0[q];
         } else {
             // common.py:152-185: normalised annulus areas, recomputed in a second pass
             const double g = linspace_at(0.0, 1.0 - 1e-7, ng, ig);
@@ -335,17 +339,20 @@ __device__ void setup_ld_role(const SetupArgs& s, int pv, int grp) {
         ak = (k0 - s.kmin - ik * s.dk) / s.dk;
     }
     const size_t j0 = (size_t)pv * s.npb + pb0;
-    // profiles at the mu nodes and their disk integrals
+    // profiles at the mu nodes, lanes over (passband, node), and their disk integrals
+    const bool tab = s.law == RR_LD_TABULATED;
+    for (int i = lane; i < gn * nz; i += 64) {
+        const int q = i / nz, iz = i - q * nz;
+        const size_t j = j0 + q;
+        s_l[i] = tab ? s.ldp_in[j * nz + iz] : ld_law(s.law, s.mu[iz], s.ldc + j * s.nldc, s.nldc);
+    }
     for (int q = 0; q < gn; ++q) {
         const size_t j = j0 + q;
-        double* l = s_l + q * nz;
         double ist;
-        if (s.law == RR_LD_TABULATED) {
-            for (int iz = lane; iz < nz; iz += 64) l[iz] = s.ldp_in[j * nz + iz];
+        if (tab) {
             ist = s.istar_in[j];
         } else {
             const double* c = s.ldc + j * s.nldc;
-            for (int iz = lane; iz < nz; iz += 64) l[iz] = ld_law(s.law, s.mu[iz], c, s.nldc);
             if (!ld_integral(s.law, c, ist)) ist = ld_trapz_wave(s.law, c, s.nldc);
         }
         if (lane == 0) s_ist[q] = ist;
