@@ -447,12 +447,15 @@ constexpr int kMaxItems = 1024;                       // (point, sample) items p
 constexpr int kMaxChunk = 64;
 
 // Bytes of dynamic LDS the flux kernel uses for these parameters.
-static inline size_t flux_lds_bytes(bool mixed, int ng, int chunk_pts, int uniform_ns, bool row_lds) {
+constexpr int kCtxWords = 5;  // per-point context of a multi-light-curve chunk (FP32)
+
+static inline size_t flux_lds_bytes(bool mixed, int ng, int chunk_pts, int uniform_ns, bool row_lds, bool multi_lc) {
     const size_t real = mixed ? 4 : 8;
     size_t b = (size_t)kTile * sizeof(uint16_t);                         // queue
     b += (size_t)kMaxChunk * sizeof(double);                             // phases of a chunk
     if (uniform_ns > 0) b += (size_t)chunk_pts * uniform_ns * real;      // item results
     if (row_lds) b += (size_t)ng * real;                                 // limb-darkening row
+    if (mixed && uniform_ns > 0 && multi_lc) b += (size_t)kCtxWords * kMaxChunk * 4;  // per-point context
     return (b + 15) / 16 * 16;
 }
 
@@ -791,6 +794,66 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
         }
         return;
     }
+    if (kMixed && f.uniform_ns > 0 && f.nlc > 1) {
+        // several light curves with one nsamples: (point, sample) items as above, with
+        // each point's light-curve context (phase, k, exposure, 1/istar, LD row) in LDS
+        const int ns = f.uniform_ns;
+        const float inv_ns = 1.0f / (float)ns;
+        float cf[10];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) cf[k] = (float)r.c[k];
+        float* s_ptc = (float*)(s_row + (f.row_lds ? f.ng : 0));
+        float* s_pk = s_ptc + kMaxChunk;
+        float* s_pext = s_pk + kMaxChunk;
+        float* s_pist = s_pext + kMaxChunk;
+        int* s_prow = (int*)(s_pist + kMaxChunk);  // passband of the point
+        const int cp = f.chunk_pts;
+        for (int c0 = 0; c0 < n; c0 += cp) {
+            const int cn = min(cp, n - c0);
+            if (wl < cn) {
+                const int it = tbase + s_q[c0 + wl];
+                const int ilc = min(max(f.lcids[it], 0), f.nlc - 1);
+                const PointCtx c = point_ctx(f, r, pv, ilc);
+                double ep;
+                s_ptc[wl] = (float)phase_of(f.times[it], c.t0v, p, inv_p, ep);
+                s_pk[wl] = (float)c.kk;
+                s_pext[wl] = (float)c.ext;
+                s_pist[wl] = (float)(1.0 / c.ist);
+                s_prow[wl] = c.ipb;
+            }
+            __syncthreads();
+            int qq = (int)(((float)wl + 0.5f) * inv_ns);  // exact i / ns for i, ns <= 1024
+            int is = wl - qq * ns;
+            const int dq = kThreads / ns, ds = kThreads - dq * ns;
+#pragma unroll 1
+            for (int i = wl; i < cn * ns; i += kThreads) {
+                const float kf = s_pk[qq];
+                const float* lrow = f.ldm32 + ((size_t)pv * f.npb + s_prow[qq]) * f.ng;
+                float d;
+                if (kf < 1.0f) {
+                    const float off = s_pext[qq] * (((float)is + 0.5f) * inv_ns - 0.5f);
+                    d = sample_small_f32(cf, kf, inv_dg, s_ptc[qq] + off, lrow, f.ng);
+                } else {
+                    d = sample_f32(cf, kf, inv_dg, s_ptc[qq], s_pext[qq], inv_ns, lrow, f.ng, is);
+                }
+                s_dep[i] = d;
+                qq += dq;
+                is += ds;
+                if (is >= ns) {
+                    is -= ns;
+                    ++qq;
+                }
+            }
+            __syncthreads();
+            if (wl < cn) {
+                float acc = 0.0f;
+                for (int k = 0; k < ns; ++k) acc += s_dep[wl * ns + k];
+                out[tbase + s_q[c0 + wl]] = Real(1.0f - acc * (s_pist[wl] * inv_ns));
+            }
+            __syncthreads();
+        }
+        return;
+    }
     // general case: one queued point per lane, per-point light-curve context
 #pragma unroll 1
     for (int q = wl; q < n; q += kThreads) {
@@ -1059,7 +1122,7 @@ static int prepare_flux(const char* who, int precision, const rr_table* tab, con
     fa.ntiles = (npt + kTile - 1) / kTile;
     nblocks = (long long)fa.ntiles * npv;
     if (nblocks > 0x7fffffffLL) return set_error(RR_ERANGE, "%s: grid too large", who);
-    lds = flux_lds_bytes(mixed, tab->ng, fa.chunk_pts, fa.uniform_ns, fa.row_lds != 0);
+    lds = flux_lds_bytes(mixed, tab->ng, fa.chunk_pts, fa.uniform_ns, fa.row_lds != 0, nlc > 1);
     fast = lc->sorted != 0;
     return RR_OK;
 }
