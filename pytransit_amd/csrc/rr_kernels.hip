@@ -445,6 +445,7 @@ constexpr int kRuns = 2;
 constexpr int kTile = kThreads * kRun * kRuns;       // 1024 times per workgroup
 constexpr int kMaxItems = 1024;                       // (point, sample) items per LDS chunk
 constexpr int kMaxChunk = 64;
+constexpr int kPartQ = 2 * kThreads;                  // queued partial-overlap samples
 
 // Bytes of dynamic LDS the flux kernel uses for these parameters.
 constexpr int kCtxWords = 5;  // per-point context of a multi-light-curve chunk (FP32)
@@ -456,6 +457,7 @@ static inline size_t flux_lds_bytes(bool mixed, int ng, int chunk_pts, int unifo
     if (uniform_ns > 0) b += (size_t)chunk_pts * uniform_ns * real;      // item results
     if (row_lds) b += (size_t)ng * real;                                 // limb-darkening row
     if (mixed && uniform_ns > 0 && multi_lc) b += (size_t)kCtxWords * kMaxChunk * 4;  // per-point context
+    if (mixed && uniform_ns > 0) b += (size_t)kPartQ * (sizeof(float) + sizeof(uint16_t));  // partial queue
     return (b + 15) / 16 * 16;
 }
 
@@ -580,13 +582,27 @@ __device__ __forceinline__ float sample_f32(const float* cf, float kf, float inv
     return ip * ap;
 }
 
-// The depth of one sample of the one-light-curve item loop: as sample_f32 for a
-// planet smaller than the star (kf < 1), where z < 1 + k gives g < 1 (no g > 1
-// branch) and the lens sort of (1, k, z) is three selects.  Returns I * A.
-__device__ __forceinline__ float sample_small_f32(const float* cf, float kf, float inv_dg, float tz, const float* lrow,
-                                                  int ng) {
+// Lens (kite) area of the unit disk and a planet of radius kf < 1 at separation z,
+// 1 - kf < z < 1 + kf: the sides (1, k, z) sorted descending are three selects
+// (common.py:5-33,52-73).
+__device__ __forceinline__ float kite_small_f32(float kf, float z) {
+    const float xs = d_max(1.0f, z), ys = d_max(kf, d_min(1.0f, z)), zs = d_min(kf, z);
+    const float sq = d_sqrt((xs + (ys + zs)) * (zs - (xs - ys)) * (zs + (xs - ys)) * (xs + (ys - zs)));
+    const float k0 = atan2_ypos_f(sq, (kf - 1.0f) * (kf + 1.0f) + z * z);
+    const float k1 = atan2_ypos_f(sq, (1.0f - kf) * (1.0f + kf) + z * z);
+    return k1 + kf * kf * k0 - 0.5f * sq;
+}
+
+// One sample of the FP32 item loops for a planet smaller than the star (kf < 1),
+// where z < 1 + k gives g < 1 (no g > 1 branch).  Returns the depth I * A of a
+// sample outside the stellar disk (0) or fully inside it (I pi k^2); for a partial
+// overlap, 1 - k < z < 1 + k, it returns the limb-darkening mean I and sets `part`,
+// and the caller multiplies by kite_small_f32(kf, z) later.
+__device__ __forceinline__ float sample_small_f32_pass1(const float* cf, float kf, float inv_dg, float tz,
+                                                        const float* lrow, int ng, float& z, bool& part) {
     const float zlim = 1.0f + kf;
-    const float z = taylor_z<float>(tz, cf);
+    z = taylor_z<float>(tz, cf);
+    part = false;
     if (!(z < zlim)) return 0.0f;
     // limb-darkening mean at g = z / (1 + k) in [0, 1): common.py:225-233
     const float x = z * (1.0f / zlim) * inv_dg;
@@ -595,12 +611,21 @@ __device__ __forceinline__ float sample_small_f32(const float* cf, float kf, flo
     const float l0 = lrow[i], l1 = lrow[i + 1];
     const float ip = fmaf(a, l1 - l0, l0);
     if (z <= 1.0f - kf) return ip * (kPiF * (kf * kf));  // planet fully inside the disk
-    // lens: the kite of (1, k, z) sorted descending (common.py:5-33,52-73)
-    const float xs = d_max(1.0f, z), ys = d_max(kf, d_min(1.0f, z)), zs = d_min(kf, z);
-    const float sq = d_sqrt((xs + (ys + zs)) * (zs - (xs - ys)) * (zs + (xs - ys)) * (xs + (ys - zs)));
-    const float k0 = atan2_ypos_f(sq, (kf - 1.0f) * (kf + 1.0f) + z * z);
-    const float k1 = atan2_ypos_f(sq, (1.0f - kf) * (1.0f + kf) + z * z);
-    return ip * (k1 + kf * kf * k0 - 0.5f * sq);
+    part = true;
+    return ip;
+}
+
+// Partial-overlap samples of an item loop, queued in LDS (item index and z) and
+// finished 64 at a time, so that the lens area runs on full waves instead of on
+// the few lanes of each round whose sample straddles the limb.
+__device__ __forceinline__ int queue_partial(bool part, int i, float z, int pq, uint16_t* s_pi, float* s_pz, int wl) {
+    const unsigned long long mk = __ballot(part);
+    if (part) {
+        const int slot = pq + __popcll(mk & ((1ull << wl) - 1ull));
+        s_pi[slot] = (uint16_t)i;
+        s_pz[slot] = z;
+    }
+    return pq + __popcll(mk);
 }
 
 template <bool kMixed, bool kFast>
@@ -646,7 +671,26 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     // ---- phase A -----------------------------------------------------------------
     if (kFast) {
         // light curves are contiguous and each is sorted in time: a run whose first
-        // and last points share a light curve lies in that light curve, in order
+        // and last points share a light curve lies in that light curve, in order.
+        // The tile as a whole first: most tiles of a long light curve hold no point
+        // of a transit window and are filled with ones at once.
+        {
+            const int last = min(tbase + kTile, f.npt) - 1;
+            const int la = f.nlc == 1 ? 0 : min(max(f.lcids[tbase], 0), f.nlc - 1);
+            const int lb = f.nlc == 1 ? 0 : min(max(f.lcids[last], 0), f.nlc - 1);
+            if (la == lb) {
+                const PointCtx c = f.nlc == 1 ? point_ctx0(f, r) : point_ctx(f, r, pv, la);
+                double ea, eb;
+                const double tca = phase_of(f.times[tbase], c.t0v, p, inv_p, ea);
+                const double tcb = phase_of(f.times[last], c.t0v, p, inv_p, eb);
+                const bool out_tile = (ea == eb) ? (tcb < c.lo || tca > c.hi)
+                                                 : ((eb == ea + 1.0) ? (tca > c.hi && tcb < c.lo) : false);
+                if (out_tile) {
+                    wave_fill<Real>(out, tbase, last + 1 - tbase, Real(1), wl);
+                    return;
+                }
+            }
+        }
         const PointCtx c1 = f.nlc == 1 ? point_ctx0(f, r) : point_ctx(f, r, pv, 0);
 #pragma unroll
         for (int m = 0; m < kRuns; ++m) {
@@ -735,6 +779,10 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     }
     __syncthreads();  // queue entries and the staged row are read by other lanes
     const float inv_dg = (float)(1.0 / f.dg);
+    // partial-overlap queue of the FP32 item loops, after the staged row and the
+    // multi-light-curve context
+    float* s_pz = (float*)(s_row + (f.row_lds ? f.ng : 0)) + (f.nlc > 1 ? kCtxWords * kMaxChunk : 0);
+    uint16_t* s_pi = (uint16_t*)(s_pz + kPartQ);
     if (f.uniform_ns > 0 && f.nlc == 1 && f.row_lds) {
         // one light curve: the per-point context is uniform
         const int ns = f.uniform_ns;
@@ -763,23 +811,47 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
             int qq = (int)(((float)wl + 0.5f) * inv_ns);  // exact i / ns for i, ns <= 1024
             int is = wl - qq * ns;
             const int dq = kThreads / ns, ds = kThreads - dq * ns;
+            if (small) {
+                // rounds of 64 items over the whole wave; partial overlaps queued
+                const int tot = cn * ns;
+                int pq = 0;
 #pragma unroll 1
-            for (int i = wl; i < cn * ns; i += kThreads) {
-                if (kMixed) {
-                    if (small) {
-                        const float tz = s_tcf[qq] + s_off[is];
-                        s_dep[i] = sample_small_f32(cf, kf, inv_dg, tz, (const float*)s_row, f.ng);
-                    } else {
-                        s_dep[i] = sample_f32(cf, kf, inv_dg, s_tcf[qq], extf, inv_ns, (const float*)s_row, f.ng, is);
+                for (int b = 0; b < tot; b += kThreads) {
+                    const int i = b + wl;
+                    bool part = false;
+                    float z = 0.0f;
+                    if (i < tot)
+                        s_dep[i] = sample_small_f32_pass1(cf, kf, inv_dg, s_tcf[qq] + s_off[is], (const float*)s_row,
+                                                          f.ng, z, part);
+                    pq = queue_partial(part, i, z, pq, s_pi, s_pz, wl);
+                    if (pq >= kThreads) {
+                        __syncthreads();
+                        pq -= kThreads;
+                        s_dep[s_pi[pq + wl]] *= kite_small_f32(kf, s_pz[pq + wl]);
+                        __syncthreads();
                     }
-                } else {
-                    s_dep[i] = sample_f64(r.c, f.dg, f.ng, c, (const double*)s_row, s_tc[qq], is);
+                    qq += dq;
+                    is += ds;
+                    if (is >= ns) {
+                        is -= ns;
+                        ++qq;
+                    }
                 }
-                qq += dq;
-                is += ds;
-                if (is >= ns) {
-                    is -= ns;
-                    ++qq;
+                __syncthreads();
+                if (wl < pq) s_dep[s_pi[wl]] *= kite_small_f32(kf, s_pz[wl]);
+            } else {
+#pragma unroll 1
+                for (int i = wl; i < cn * ns; i += kThreads) {
+                    if (kMixed)
+                        s_dep[i] = sample_f32(cf, kf, inv_dg, s_tcf[qq], extf, inv_ns, (const float*)s_row, f.ng, is);
+                    else
+                        s_dep[i] = sample_f64(r.c, f.dg, f.ng, c, (const double*)s_row, s_tc[qq], is);
+                    qq += dq;
+                    is += ds;
+                    if (is >= ns) {
+                        is -= ns;
+                        ++qq;
+                    }
                 }
             }
             __syncthreads();
@@ -825,24 +897,42 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
             int qq = (int)(((float)wl + 0.5f) * inv_ns);  // exact i / ns for i, ns <= 1024
             int is = wl - qq * ns;
             const int dq = kThreads / ns, ds = kThreads - dq * ns;
+            const int tot = cn * ns;
+            int pq = 0;
 #pragma unroll 1
-            for (int i = wl; i < cn * ns; i += kThreads) {
-                const float kf = s_pk[qq];
-                const float* lrow = f.ldm32 + ((size_t)pv * f.npb + s_prow[qq]) * f.ng;
-                float d;
-                if (kf < 1.0f) {
-                    const float off = s_pext[qq] * (((float)is + 0.5f) * inv_ns - 0.5f);
-                    d = sample_small_f32(cf, kf, inv_dg, s_ptc[qq] + off, lrow, f.ng);
-                } else {
-                    d = sample_f32(cf, kf, inv_dg, s_ptc[qq], s_pext[qq], inv_ns, lrow, f.ng, is);
+            for (int b = 0; b < tot; b += kThreads) {
+                const int i = b + wl;
+                bool part = false;
+                float z = 0.0f;
+                if (i < tot) {
+                    const float kf = s_pk[qq];
+                    const float* lrow = f.ldm32 + ((size_t)pv * f.npb + s_prow[qq]) * f.ng;
+                    if (kf < 1.0f) {
+                        const float off = s_pext[qq] * (((float)is + 0.5f) * inv_ns - 0.5f);
+                        s_dep[i] = sample_small_f32_pass1(cf, kf, inv_dg, s_ptc[qq] + off, lrow, f.ng, z, part);
+                    } else {
+                        s_dep[i] = sample_f32(cf, kf, inv_dg, s_ptc[qq], s_pext[qq], inv_ns, lrow, f.ng, is);
+                    }
                 }
-                s_dep[i] = d;
+                pq = queue_partial(part, i, z, pq, s_pi, s_pz, wl);
+                if (pq >= kThreads) {
+                    __syncthreads();
+                    pq -= kThreads;
+                    const int j = s_pi[pq + wl];
+                    s_dep[j] *= kite_small_f32(s_pk[(int)(((float)j + 0.5f) * inv_ns)], s_pz[pq + wl]);
+                    __syncthreads();
+                }
                 qq += dq;
                 is += ds;
                 if (is >= ns) {
                     is -= ns;
                     ++qq;
                 }
+            }
+            __syncthreads();
+            if (wl < pq) {
+                const int j = s_pi[wl];
+                s_dep[j] *= kite_small_f32(s_pk[(int)(((float)j + 0.5f) * inv_ns)], s_pz[wl]);
             }
             __syncthreads();
             if (wl < cn) {
