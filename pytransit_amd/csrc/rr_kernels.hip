@@ -457,7 +457,7 @@ static inline size_t flux_lds_bytes(bool mixed, int ng, int chunk_pts, int unifo
     if (uniform_ns > 0) b += (size_t)chunk_pts * uniform_ns * real;      // item results
     if (row_lds) b += (size_t)ng * real;                                 // limb-darkening row
     if (mixed && uniform_ns > 0 && multi_lc) b += (size_t)kCtxWords * kMaxChunk * 4;  // per-point context
-    if (mixed && uniform_ns > 0) b += (size_t)kPartQ * (sizeof(float) + sizeof(uint16_t));  // partial queue
+    if (uniform_ns > 0) b += (size_t)kPartQ * (real + sizeof(uint16_t));                 // partial queue
     return (b + 15) / 16 * 16;
 }
 
@@ -582,6 +582,32 @@ __device__ __forceinline__ float sample_f32(const float* cf, float kf, float inv
     return ip * ap;
 }
 
+// sample_f64 in two passes for the item loop with a queue of partial overlaps:
+// returns the sample's (istar - I*A)/istar unless the planet straddles the limb,
+// |1 - k| < z < 1 + k, where it returns I and sets `part`; sample_f64_finish then
+// completes it with the same operations as sample_f64.
+__device__ __forceinline__ double sample_f64_pass1(const double* cc, double dg, int ng, const PointCtx& c,
+                                                   const double* lrow, double tc, int is, double& z, bool& part) {
+    const double off = c.ext * ((is + 1 - 0.5) / c.ns - 0.5);
+    z = taylor_z<double>(tc + off, cc);
+    part = false;
+    if (!(z < 1.0 + c.kk)) return 1.0;
+    const double ip = interp_ld(z / (1.0 + c.kk), dg, lrow, ng);
+    if (d_abs(1.0 - c.kk) < z) {
+        part = true;
+        return ip;
+    }
+    double ap, kap;
+    ccia_kite<double>(1.0, c.kk, z, ap, kap);
+    return (c.ist - ip * ap) / c.ist;
+}
+
+__device__ __forceinline__ double sample_f64_finish(double ip, const PointCtx& c, double z) {
+    double ap, kap;
+    ccia_kite<double>(1.0, c.kk, z, ap, kap);
+    return (c.ist - ip * ap) / c.ist;
+}
+
 // Lens (kite) area of the unit disk and a planet of radius kf < 1 at separation z,
 // 1 - kf < z < 1 + kf: the sides (1, k, z) sorted descending are three selects
 // (common.py:5-33,52-73).
@@ -618,7 +644,8 @@ __device__ __forceinline__ float sample_small_f32_pass1(const float* cf, float k
 // Partial-overlap samples of an item loop, queued in LDS (item index and z) and
 // finished 64 at a time, so that the lens area runs on full waves instead of on
 // the few lanes of each round whose sample straddles the limb.
-__device__ __forceinline__ int queue_partial(bool part, int i, float z, int pq, uint16_t* s_pi, float* s_pz, int wl) {
+template <typename T>
+__device__ __forceinline__ int queue_partial(bool part, int i, T z, int pq, uint16_t* s_pi, T* s_pz, int wl) {
     const unsigned long long mk = __ballot(part);
     if (part) {
         const int slot = pq + __popcll(mk & ((1ull << wl) - 1ull));
@@ -781,8 +808,9 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     const float inv_dg = (float)(1.0 / f.dg);
     // partial-overlap queue of the FP32 item loops, after the staged row and the
     // multi-light-curve context
-    float* s_pz = (float*)(s_row + (f.row_lds ? f.ng : 0)) + (f.nlc > 1 ? kCtxWords * kMaxChunk : 0);
+    Real* s_pz = (Real*)(s_row + (f.row_lds ? f.ng : 0)) + (kMixed && f.nlc > 1 ? kCtxWords * kMaxChunk : 0);
     uint16_t* s_pi = (uint16_t*)(s_pz + kPartQ);
+    float* s_pzf = (float*)s_pz;  // the FP32 loops' view
     if (f.uniform_ns > 0 && f.nlc == 1 && f.row_lds) {
         // one light curve: the per-point context is uniform
         const int ns = f.uniform_ns;
@@ -823,11 +851,11 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                     if (i < tot)
                         s_dep[i] = sample_small_f32_pass1(cf, kf, inv_dg, s_tcf[qq] + s_off[is], (const float*)s_row,
                                                           f.ng, z, part);
-                    pq = queue_partial(part, i, z, pq, s_pi, s_pz, wl);
+                    pq = queue_partial<float>(part, i, z, pq, s_pi, s_pzf, wl);
                     if (pq >= kThreads) {
                         __syncthreads();
                         pq -= kThreads;
-                        s_dep[s_pi[pq + wl]] *= kite_small_f32(kf, s_pz[pq + wl]);
+                        s_dep[s_pi[pq + wl]] *= kite_small_f32(kf, s_pzf[pq + wl]);
                         __syncthreads();
                     }
                     qq += dq;
@@ -838,14 +866,41 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                     }
                 }
                 __syncthreads();
-                if (wl < pq) s_dep[s_pi[wl]] *= kite_small_f32(kf, s_pz[wl]);
+                if (wl < pq) s_dep[s_pi[wl]] *= kite_small_f32(kf, s_pzf[wl]);
+            } else if (!kMixed) {
+                const int tot = cn * ns;
+                int pq = 0;
+#pragma unroll 1
+                for (int b = 0; b < tot; b += kThreads) {
+                    const int i = b + wl;
+                    bool part = false;
+                    double z = 0.0;
+                    if (i < tot)
+                        s_dep[i] = sample_f64_pass1(r.c, f.dg, f.ng, c, (const double*)s_row, s_tc[qq], is, z, part);
+                    pq = queue_partial<Real>(part, i, z, pq, s_pi, s_pz, wl);
+                    if (pq >= kThreads) {
+                        __syncthreads();
+                        pq -= kThreads;
+                        const int j = s_pi[pq + wl];
+                        s_dep[j] = sample_f64_finish(s_dep[j], c, s_pz[pq + wl]);
+                        __syncthreads();
+                    }
+                    qq += dq;
+                    is += ds;
+                    if (is >= ns) {
+                        is -= ns;
+                        ++qq;
+                    }
+                }
+                __syncthreads();
+                if (wl < pq) {
+                    const int j = s_pi[wl];
+                    s_dep[j] = sample_f64_finish(s_dep[j], c, s_pz[wl]);
+                }
             } else {
 #pragma unroll 1
                 for (int i = wl; i < cn * ns; i += kThreads) {
-                    if (kMixed)
-                        s_dep[i] = sample_f32(cf, kf, inv_dg, s_tcf[qq], extf, inv_ns, (const float*)s_row, f.ng, is);
-                    else
-                        s_dep[i] = sample_f64(r.c, f.dg, f.ng, c, (const double*)s_row, s_tc[qq], is);
+                    s_dep[i] = sample_f32(cf, kf, inv_dg, s_tcf[qq], extf, inv_ns, (const float*)s_row, f.ng, is);
                     qq += dq;
                     is += ds;
                     if (is >= ns) {
@@ -914,12 +969,12 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                         s_dep[i] = sample_f32(cf, kf, inv_dg, s_ptc[qq], s_pext[qq], inv_ns, lrow, f.ng, is);
                     }
                 }
-                pq = queue_partial(part, i, z, pq, s_pi, s_pz, wl);
+                pq = queue_partial<float>(part, i, z, pq, s_pi, s_pzf, wl);
                 if (pq >= kThreads) {
                     __syncthreads();
                     pq -= kThreads;
                     const int j = s_pi[pq + wl];
-                    s_dep[j] *= kite_small_f32(s_pk[(int)(((float)j + 0.5f) * inv_ns)], s_pz[pq + wl]);
+                    s_dep[j] *= kite_small_f32(s_pk[(int)(((float)j + 0.5f) * inv_ns)], s_pzf[pq + wl]);
                     __syncthreads();
                 }
                 qq += dq;
@@ -932,7 +987,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
             __syncthreads();
             if (wl < pq) {
                 const int j = s_pi[wl];
-                s_dep[j] *= kite_small_f32(s_pk[(int)(((float)j + 0.5f) * inv_ns)], s_pz[wl]);
+                s_dep[j] *= kite_small_f32(s_pk[(int)(((float)j + 0.5f) * inv_ns)], s_pzf[wl]);
             }
             __syncthreads();
             if (wl < cn) {
