@@ -156,7 +156,8 @@ __global__ void __launch_bounds__(64) ma_orbit_kernel(int npv, const double* P, 
     const int q = inrange ? pv : 0;
     const double k0 = K[(size_t)q * kcols];
     PvRec* r = rec + q;
-    orbit_group(pv, inrange, P[q], A[q], I[q], E[q], W[q], k0, r->c, &r->bt1);
+    __shared__ double sm[kOrbitLds];
+    orbit_group(pv, inrange, P[q], A[q], I[q], E[q], W[q], k0, r->c, &r->bt1, sm);
     if (inrange && j == 0) {
         r->p = P[q];
         r->inv_p = 1.0 / P[q];

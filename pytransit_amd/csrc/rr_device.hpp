@@ -332,25 +332,31 @@ __device__ __forceinline__ double ld_trapz_segment(int law, const double* c, int
 // the first and fourth contacts.  Writes coefficients[10] and the window.
 constexpr int kOrbitGroup = 8;
 
+constexpr int kOrbitLds = kOrbitGroup * (8 + 8 + 10 + 2);  // doubles of LDS per block
+
+// `sm` is block-shared storage of kOrbitLds doubles.
 __device__ __forceinline__ void orbit_group(int pv, bool valid, double p, double a, double inc, double e, double w,
-                                            double kw, double* out_c, double* out_win) {
-    __shared__ double s_x[8][8], s_y[8][8], s_c[8][10], s_w[8][2];
+                                            double kw, double* out_c, double* out_win, double* sm) {
     const int sub = threadIdx.x / kOrbitGroup, j = threadIdx.x % kOrbitGroup;
+    double* s_x = sm + sub * 8;
+    double* s_y = sm + kOrbitGroup * 8 + sub * 8;
+    double* s_c = sm + kOrbitGroup * 16 + sub * 10;
+    double* s_w = sm + kOrbitGroup * 26 + sub * 2;
     if (valid && j < 7) {
         double x, y;
         kepler_xy((j - 3) * 2e-2, p, a, cos(inc), e, w, x, y);
-        s_x[sub][j] = x;
-        s_y[sub][j] = y;
+        s_x[j] = x;
+        s_y[j] = y;
     }
     __syncthreads();
-    if (valid && j == 0) taylor_from_stencil(s_x[sub], s_y[sub], s_c[sub]);
+    if (valid && j == 0) taylor_from_stencil(s_x, s_y, s_c);
     __syncthreads();
-    if (valid && j < 2) s_w[sub][j] = find_contact_point(kw, j == 0 ? 1 : 4, s_c[sub]);
+    if (valid && j < 2) s_w[j] = find_contact_point(kw, j == 0 ? 1 : 4, s_c);
     __syncthreads();
     if (valid) {
-        out_c[j] = s_c[sub][j];
-        if (j < 2) out_c[8 + j] = s_c[sub][8 + j];
-        if (j < 2) out_win[j] = s_w[sub][j];
+        out_c[j] = s_c[j];
+        if (j < 2) out_c[8 + j] = s_c[8 + j];
+        if (j < 2) out_win[j] = s_w[j];
     }
 }
 

@@ -134,7 +134,8 @@ __global__ void __launch_bounds__(64) orbit_kernel(int npv, const double* __rest
     const int pv = blockIdx.x * kOrbitGroup + threadIdx.x / kOrbitGroup;
     const bool valid = pv < npv;
     const int q = valid ? pv : 0;
-    orbit_group(pv, valid, P[q], A[q], I[q], E[q], Wp[q], K0[q], coeffs + (size_t)q * 10, win + (size_t)q * 2);
+    __shared__ double sm[kOrbitLds];
+    orbit_group(pv, valid, P[q], A[q], I[q], E[q], Wp[q], K0[q], coeffs + (size_t)q * 10, win + (size_t)q * 2, sm);
 }
 
 // ------------------------------------------------------------------------------------
@@ -157,7 +158,7 @@ struct SetupArgs {
 __device__ __forceinline__ int lc0_pb(const int32_t* pbids, int npb) { return min(max(pbids[0], 0), npb - 1); }
 __device__ __forceinline__ int lc0_ep(const int32_t* epids, int nep) { return min(max(epids[0], 0), nep - 1); }
 
-__device__ void setup_orbit_role(const SetupArgs& s, int blk) {
+__device__ void setup_orbit_role(const SetupArgs& s, int blk, double* sm) {
     const int pv = blk * kOrbitGroup + threadIdx.x / kOrbitGroup;
     const int j = threadIdx.x % kOrbitGroup;
     const bool inrange = pv < s.npv;
@@ -166,7 +167,7 @@ __device__ void setup_orbit_role(const SetupArgs& s, int blk) {
     const bool valid = inrange && !(isnan(a) || a <= 1.0 || e < 0.0);
     const double k0 = s.k[(size_t)q * s.kcols];
     PvRec* rec = s.rec + q;
-    orbit_group(pv, valid, s.p[q], a, s.inc[q], e, s.w[q], k0, rec->c, &rec->bt1);
+    orbit_group(pv, valid, s.p[q], a, s.inc[q], e, s.w[q], k0, rec->c, &rec->bt1, sm);
     if (inrange && j == 0) {
         rec->p = s.p[q];
         rec->inv_p = 1.0 / s.p[q];
@@ -183,7 +184,7 @@ __device__ void setup_orbit_role(const SetupArgs& s, int blk) {
 // group share each table read (every passband of a pv uses the rows of k0,
 // model_full.py:47-51).
 constexpr int kLdGroup = 8;
-constexpr int kLdLds = 1024;  // doubles of profile storage per block
+constexpr int kLdLds = 1024;  // doubles of profile storage per block (>= kMaxNz)
 
 __host__ __device__ inline int ld_group_size(int nz) {
     const int g = kLdLds / (nz > 0 ? nz : 1);
@@ -321,9 +322,56 @@ __device__ __forceinline__ void ld_means_single(const SetupArgs& s, size_t j, in
     }
 }
 
-__device__ void setup_ld_role(const SetupArgs& s, int pv, int grp) {
-    __shared__ double s_l[kLdLds];
-    __shared__ double s_ist[kLdGroup];
+// One passband inside the table with 16-byte table loads: lane task t covers kV
+// consecutive g of one of the two rows (t < nv: row ik, else ik + 1) over all mu
+// nodes, so each load instruction moves up to 1 KB; the per-row sums meet in LDS.
+// Same sums in the same order as ld_means_single.  Needs ng % kV == 0, a 16-byte
+// aligned table and nz + 2 ng doubles of `sm`.
+template <typename T>
+__device__ __forceinline__ void ld_means_vec(const SetupArgs& s, size_t j, int ik, double ak, const double* s_l,
+                                             double* s_part) {
+    typedef T vec __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int kV = 16 / sizeof(T);
+    constexpr int kB = 10;
+    const int lane = threadIdx.x;
+    const int nz = s.nz, ng = s.ng;
+    const int nv = ng / kV;
+    const T* W = std::is_same<T, float>::value ? (const T*)s.W32 : (const T*)s.W;
+    const size_t base = (size_t)ik * nz * ng;
+    for (int t = lane; t < 2 * nv; t += 64) {
+        const int row = t >= nv ? 1 : 0;
+        const int v = t - row * nv;
+        const T* src = W + base + (size_t)row * nz * ng + (size_t)v * kV;
+        double acc[kV];
+#pragma unroll
+        for (int e = 0; e < kV; ++e) acc[e] = 0.0;
+        for (int iz0 = 0; iz0 < nz; iz0 += kB) {
+            vec w[kB];
+#pragma unroll
+            for (int u = 0; u < kB; ++u) w[u] = *(const vec*)(src + (size_t)min(iz0 + u, nz - 1) * ng);
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                if (iz0 + u < nz) {
+                    const double l = s_l[iz0 + u];
+#pragma unroll
+                    for (int e = 0; e < kV; ++e) acc[e] += (double)w[u][e] * l;
+                }
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < kV; ++e) s_part[row * ng + v * kV + e] = acc[e];
+    }
+    __syncthreads();
+    for (int ig = lane; ig < ng; ig += 64) {
+        const double v = (1.0 - ak) * s_part[ig] + ak * s_part[ng + ig];
+        if (s.ldm32) s.ldm32[j * ng + ig] = (float)v;
+        else s.ldm64[j * ng + ig] = v;
+    }
+}
+
+__device__ void setup_ld_role(const SetupArgs& s, int pv, int grp, double* sm) {
+    double* s_l = sm;                // kLdLds
+    double* s_ist = sm + kLdLds;     // kLdGroup
     const int lane = threadIdx.x;
     const int nz = s.nz, ng = s.ng;
     const int gs = ld_group_size(nz);
@@ -366,23 +414,38 @@ __device__ void setup_ld_role(const SetupArgs& s, int pv, int grp) {
     }
     // limb-darkening means: lerp in k of two table rows, or exact 2-D weights
     // when k0 is outside [kmin, kmax]; model_full.py:47-55
-    if (gn == 1 && in_table) ld_means_single(s, j0, ik, ak, s_l);
+    const bool mixed = s.ldm32 != nullptr;
+    const void* wt = mixed ? (const void*)s.W32 : (const void*)s.W;
+    const int kv = mixed ? 4 : 2;
+    const bool vec_ok = wt != nullptr && ng % kv == 0 && ((uintptr_t)wt & 15) == 0 && nz + 2 * ng <= kLdLds;
+    if (gn == 1 && in_table && vec_ok) {
+        if (mixed) ld_means_vec<float>(s, j0, ik, ak, s_l, s_l + nz);
+        else ld_means_vec<double>(s, j0, ik, ak, s_l, s_l + nz);
+    } else if (gn == 1 && in_table) ld_means_single(s, j0, ik, ak, s_l);
     else if (gn == 1) ld_means<1>(s, pv, j0, 1, in_table, ik, ak, k0, s_l);
     else ld_means<kLdGroup>(s, pv, j0, gn, in_table, ik, ak, k0, s_l);
 }
 
+// Both roles share one LDS buffer, so a setup block holds only the larger of the two.
+constexpr int kSetupLds = kLdLds + kLdGroup > kOrbitLds ? kLdLds + kLdGroup : kOrbitLds;
+
 __device__ __forceinline__ void setup_body(const SetupArgs& s, int blk) {
+    __shared__ double sm[kSetupLds];
     if (blk < s.n_orbit_blocks) {
-        setup_orbit_role(s, blk);
+        setup_orbit_role(s, blk, sm);
     } else {
         const int b = blk - s.n_orbit_blocks;
         const int ngrp = (s.npb + ld_group_size(s.nz) - 1) / ld_group_size(s.nz);
         const int pv = b / ngrp;
-        setup_ld_role(s, pv, b - pv * ngrp);
+        setup_ld_role(s, pv, b - pv * ngrp, sm);
     }
 }
 
-__global__ void __launch_bounds__(64) rr_setup_kernel(SetupArgs s) { setup_body(s, blockIdx.x); }
+// Five waves per SIMD: with the block's 8.3 KB of LDS, 19 blocks fit a CU, so the
+// 4096 + 512 blocks of a 4096-pv setup are resident at once on 256 CUs.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) rr_setup_kernel(SetupArgs s) {
+    setup_body(s, blockIdx.x);
+}
 
 // Several requests in one launch: item i owns blocks [start[i], start[i+1]).  The
 // item index is wave-uniform, so its arguments are scalar loads.
@@ -400,7 +463,7 @@ __device__ __forceinline__ int batch_item(const int* start, int n, int bid) {
     return i;
 }
 
-__global__ void __launch_bounds__(64) rr_setup_batch_kernel(SetupBatch b) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) rr_setup_batch_kernel(SetupBatch b) {
     const int i = batch_item(b.start, b.n, blockIdx.x);
     setup_body(b.a[i], blockIdx.x - b.start[i]);
 }
