@@ -360,6 +360,19 @@ __device__ __forceinline__ void orbit_group(int pv, bool valid, double p, double
     }
 }
 
+// Workgroups are dealt to the 8 XCDs of the chip round-robin (XCD = blockIdx % 8).
+// Block b is given work item xcd_remap(b, n) instead, so that each XCD works through
+// one contiguous eighth of the n items: a pattern in the item order (a transit in the
+// first tile of every pv, say) is spread over all XCDs rather than landing on the few
+// that the round-robin assigns it to, and an XCD's L2 keeps the pvs it works on.
+// A bijection of [0, n).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    const int x = b % kXcds, j = b / kXcds;
+    const int per = n / kXcds, r = n % kXcds;
+    return x * per + min(x, r) + j;
+}
+
 // Sum over a wavefront (64 lanes) by butterfly shuffles; every lane gets the total.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

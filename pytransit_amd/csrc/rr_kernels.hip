@@ -860,6 +860,15 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     }
     const int n = count;
     if (n == 0) return;
+    // one light curve with one nsamples: the first chunk's times are loaded here, in
+    // flight together with the limb-darkening row staged below, and each chunk loads
+    // the next chunk's times before its items (no exposed load latency per chunk)
+    const bool uniform_lc = f.uniform_ns > 0 && f.nlc == 1 && f.row_lds;
+    double tnext = 0.0;
+    if (uniform_lc) {
+        __syncthreads();  // the queue entries of other lanes
+        if (wl < min(f.chunk_pts, n)) tnext = f.times[tbase + s_q[wl]];
+    }
 
     // ---- phase B -----------------------------------------------------------------
     if (f.row_lds) {
@@ -874,7 +883,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     Real* s_pz = (Real*)(s_row + (f.row_lds ? f.ng : 0)) + (kMixed && f.nlc > 1 ? kCtxWords * kMaxChunk : 0);
     uint16_t* s_pi = (uint16_t*)(s_pz + kPartQ);
     float* s_pzf = (float*)s_pz;  // the FP32 loops' view
-    if (f.uniform_ns > 0 && f.nlc == 1 && f.row_lds) {
+    if (uniform_lc) {
         // one light curve: the per-point context is uniform
         const int ns = f.uniform_ns;
         const PointCtx c = point_ctx0(f, r);
@@ -891,9 +900,11 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
         const int cp = f.chunk_pts;
         for (int c0 = 0; c0 < n; c0 += cp) {
             const int cn = min(cp, n - c0);
+            const double tcur = tnext;
+            if (wl < cp && wl < n - (c0 + cp)) tnext = f.times[tbase + s_q[c0 + cp + wl]];
             if (wl < cn) {
                 double ep;
-                const double tc = phase_of(f.times[tbase + s_q[c0 + wl]], c.t0v, p, inv_p, ep);
+                const double tc = phase_of(tcur, c.t0v, p, inv_p, ep);
                 if (kMixed) s_tcf[wl] = (float)tc;
                 else s_tc[wl] = tc;
             }
@@ -1092,7 +1103,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
 
 template <bool kMixed, bool kFast>
 __global__ void __launch_bounds__(kThreads) rr_flux_kernel(FluxArgs f) {
-    flux_body<kMixed, kFast>(f, blockIdx.x);
+    flux_body<kMixed, kFast>(f, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 struct FluxBatch {
@@ -1103,8 +1114,9 @@ struct FluxBatch {
 
 template <bool kMixed, bool kFast>
 __global__ void __launch_bounds__(kThreads) rr_flux_batch_kernel(FluxBatch b) {
-    const int i = batch_item(b.start, b.n, blockIdx.x);
-    flux_body<kMixed, kFast>(b.a[i], blockIdx.x - b.start[i]);
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int i = batch_item(b.start, b.n, bid);
+    flux_body<kMixed, kFast>(b.a[i], bid - b.start[i]);
 }
 
 static_assert(sizeof(SetupBatch) <= 4096 && sizeof(FluxBatch) <= 4096, "kernel arguments above 4 KB");
