@@ -172,8 +172,9 @@ __global__ void __launch_bounds__(64) ma_orbit_kernel(int npv, const double* P, 
 
 // ma_quadratic_nb.py:557-594
 __global__ void __launch_bounds__(kMaThreads) ma_flux_kernel(MaArgs f) {
-    const int pv = blockIdx.x / f.ntiles;
-    const int it = (blockIdx.x - pv * f.ntiles) * kMaThreads + threadIdx.x;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int pv = bid / f.ntiles;
+    const int it = (bid - pv * f.ntiles) * kMaThreads + threadIdx.x;
     if (it >= f.npt) return;
     const PvRec& r = f.rec[pv];
     double* out = f.flux + (size_t)pv * f.npt + it;

@@ -411,8 +411,9 @@ __global__ void __launch_bounds__(kTsThreads) ts_flux_kernel(TsArgs s, void* flu
     Real* s_iist = s_kd + PC;                       // 1 / istar
     __shared__ int s_any;
 
-    const int tile = blockIdx.x % ntiles;
-    const int pv = blockIdx.x / ntiles;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int tile = bid % ntiles;
+    const int pv = bid / ntiles;
     const int t0l = (tile * kTsThreads + (int)threadIdx.x) * TPL;  // first time of the lane
     const int nt = max(0, min(TPL, s.npt - t0l));                   // its valid times
     const TsRec& r = s.rec[pv];
