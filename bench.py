@@ -22,7 +22,7 @@ Further legs, reported beside the headline in the same JSON line:
   batched (4 slots, up to 16 requests per launch) and unbatched (one at a time),
   with throughput and p50/p99 latency.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu] [--no-serve] [--no-extra]
+Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 3] [--no-cpu] [--no-serve] [--no-extra]
 """
 import argparse
 import json
@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--steps', type=int, default=100)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--cpu-reps', type=int, default=5)
@@ -90,6 +90,10 @@ def flux_kernel_bytes(npv, npt, ng, out_bytes):
 
 
 def time_path(tm, args_dev, steps, warmup):
+    """Seconds per evaluation over `steps` back-to-back evaluations, then the mean
+    flux-kernel duration (ms) from HIP events around each flux launch of a second
+    run of `steps` evaluations.  The two are separate because an event pair around
+    every launch adds its own gaps between the kernels of a step."""
     import torch
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     for _ in range(warmup):
@@ -98,9 +102,12 @@ def time_path(tm, args_dev, steps, warmup):
     t0 = time.perf_counter()
     out = None
     for s in range(steps):
-        out = tm._evaluate_device(*args_dev, flux_events=ev[s])
+        out = tm._evaluate_device(*args_dev)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    for s in range(steps):
+        tm._evaluate_device(*args_dev, flux_events=ev[s])
+    torch.cuda.synchronize()
     kern_ms = statistics.mean(a.elapsed_time(b) for a, b in ev)
     return wall / steps, kern_ms, out
 
