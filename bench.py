@@ -18,7 +18,7 @@ Further legs, reported beside the headline in the same JSON line:
 * ``ts_many_pb``: TSModel, 1,024 pv x 5,000 times x 512 passbands (mixed and FP64);
 * ``gradient``: fused log-likelihood with its adjoint gradient, 2,048 pv x 10,000
   times (mixed and FP64);
-* ``serving``: a Poisson stream of 300 fit requests through the request server,
+* ``serving``: a Poisson stream of 1000 fit requests through the request server,
   batched (4 slots, up to 16 requests per launch) and unbatched (one at a time),
   with throughput and p50/p99 latency.
 
@@ -48,7 +48,7 @@ def parse():
     ap.add_argument('--no-serve', action='store_true', help='skip the serving leg')
     ap.add_argument('--no-extra', action='store_true',
                     help='skip the many-passband TSModel and likelihood-gradient legs')
-    ap.add_argument('--serve-requests', type=int, default=300)
+    ap.add_argument('--serve-requests', type=int, default=1000)
     ap.add_argument('--serve-rate', type=float, default=2000.0, help='Poisson arrival rate, requests/s')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also launch a plain 256 MiB device copy (the calibration kernel of the HBM counter runs)')
