@@ -513,16 +513,6 @@ constexpr int kPartQ = 2 * kThreads;                  // queued partial-overlap 
 // Bytes of dynamic LDS the flux kernel uses for these parameters.
 constexpr int kCtxWords = 5;  // per-point context of a multi-light-curve chunk (FP32)
 
-static inline size_t flux_lds_bytes(bool mixed, int ng, int chunk_pts, int uniform_ns, bool row_lds, bool multi_lc) {
-    const size_t real = mixed ? 4 : 8;
-    size_t b = (size_t)kTile * sizeof(uint16_t);                         // queue
-    b += (size_t)kMaxChunk * sizeof(double);                             // phases of a chunk
-    if (uniform_ns > 0) b += (size_t)chunk_pts * uniform_ns * real;      // item results
-    if (row_lds) b += (size_t)ng * real;                                 // limb-darkening row
-    if (mixed && uniform_ns > 0 && multi_lc) b += (size_t)kCtxWords * kMaxChunk * 4;  // per-point context
-    if (uniform_ns > 0) b += (size_t)kPartQ * (real + sizeof(uint16_t));                 // partial queue
-    return (b + 15) / 16 * 16;
-}
 
 // Epoch and phase of time t (the division-free form of model_full.py:88-89).
 __device__ __forceinline__ double phase_of(double t, double t0, double p, double inv_p, double& epoch) {
@@ -590,6 +580,12 @@ struct PointCtx {
     double ext, kk, ist, t0v, lo, hi;
 };
 
+// A tile of sorted light curves that spans several light curves (at most
+// kTileLcs) stages their contexts in LDS: one dependent-load chain for all.  The
+// table lives in the chunk-phase area, which only the one-light-curve loops use.
+constexpr int kTileLcs = 8;
+static_assert(kTileLcs * sizeof(PointCtx) <= kMaxChunk * sizeof(double), "light-curve table");
+
 __device__ __forceinline__ PointCtx point_ctx(const FluxArgs& f, const PvRec& r, int pv, int ilc) {
     PointCtx c;
     c.ipb = min(max(f.pbids[ilc], 0), f.npb - 1);
@@ -602,6 +598,18 @@ __device__ __forceinline__ PointCtx point_ctx(const FluxArgs& f, const PvRec& r,
     c.lo = r.bt1 - margin;
     c.hi = r.bt4 + margin;
     return c;
+}
+
+// Bytes of dynamic LDS the flux kernel uses for these parameters.
+static inline size_t flux_lds_bytes(bool mixed, int ng, int chunk_pts, int uniform_ns, bool row_lds, bool multi_lc) {
+    const size_t real = mixed ? 4 : 8;
+    size_t b = (size_t)kTile * sizeof(uint16_t);                         // queue
+    b += (size_t)kMaxChunk * sizeof(double);                             // phases of a chunk
+    if (uniform_ns > 0) b += (size_t)chunk_pts * uniform_ns * real;      // item results
+    if (row_lds) b += (size_t)ng * real;                                 // limb-darkening row
+    if (mixed && uniform_ns > 0 && multi_lc) b += (size_t)kCtxWords * kMaxChunk * 4;  // per-point context
+    if (uniform_ns > 0) b += (size_t)kPartQ * (real + sizeof(uint16_t));                 // partial queue
+    return (b + 15) / 16 * 16;
 }
 
 // Light curve 0's context from the per-pv record (one light curve: no dependent loads).
@@ -751,6 +759,24 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
         return;
     }
     const double p = r.p, inv_p = r.inv_p;
+    // sorted light curves: the tile's points lie in light curves la0..lb0
+    // (lcids are non-decreasing); up to kTileLcs of them are staged in LDS
+    PointCtx* s_lc = (PointCtx*)s_tc;
+    int la0 = 0;
+    bool lc_lds = false;
+    if (kFast && f.nlc > 1) {
+        la0 = min(max(f.lcids[tbase], 0), f.nlc - 1);
+        const int lb0 = min(max(f.lcids[min(tbase + kTile, f.npt) - 1], 0), f.nlc - 1);
+        lc_lds = lb0 != la0 && lb0 - la0 < kTileLcs;
+        if (lc_lds) {
+            if (wl <= lb0 - la0) s_lc[wl] = point_ctx(f, r, pv, la0 + wl);
+            __syncthreads();
+        }
+    }
+    // (the clamp keeps a caller that flags unsorted data as sorted inside the table)
+    auto ctx_of = [&](int ilc) -> PointCtx {
+        return lc_lds ? s_lc[min(max(ilc - la0, 0), kTileLcs - 1)] : point_ctx(f, r, pv, ilc);
+    };
     int count = 0;
     auto push = [&](bool pred, int it) {
         const unsigned long long mk = __ballot(pred);
@@ -769,7 +795,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
             const int la = f.nlc == 1 ? 0 : min(max(f.lcids[tbase], 0), f.nlc - 1);
             const int lb = f.nlc == 1 ? 0 : min(max(f.lcids[last], 0), f.nlc - 1);
             if (la == lb) {
-                const PointCtx c = f.nlc == 1 ? point_ctx0(f, r) : point_ctx(f, r, pv, la);
+                const PointCtx c = f.nlc == 1 ? point_ctx0(f, r) : ctx_of(la);
                 double ea, eb;
                 const double tca = phase_of(f.times[tbase], c.t0v, p, inv_p, ea);
                 const double tcb = phase_of(f.times[last], c.t0v, p, inv_p, eb);
@@ -781,7 +807,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                 }
             }
         }
-        const PointCtx c1 = f.nlc == 1 ? point_ctx0(f, r) : point_ctx(f, r, pv, 0);
+        const PointCtx c1 = f.nlc == 1 ? point_ctx0(f, r) : ctx_of(la0);
 #pragma unroll
         for (int m = 0; m < kRuns; ++m) {
             bool all_out = true, one_lc = true, one_ep = false;
@@ -791,7 +817,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                 const int la = min(max(f.lcids[its[m]], 0), f.nlc - 1);
                 const int lb = min(max(f.lcids[its[m] + nrun[m] - 1], 0), f.nlc - 1);
                 one_lc = la == lb;
-                c = point_ctx(f, r, pv, la);
+                c = ctx_of(la);
             }
             if (nrun[m] > 0) {
                 all_out = false;
@@ -827,7 +853,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                     if (one_ep) {
                         tc = tv[j] - base;  // model_full.py:89 with the run's epoch
                     } else {
-                        if (!one_lc) cc = point_ctx(f, r, pv, min(max(f.lcids[it], 0), f.nlc - 1));
+                        if (!one_lc) cc = ctx_of(min(max(f.lcids[it], 0), f.nlc - 1));
                         double ep;
                         tc = phase_of(tv[j], cc.t0v, p, inv_p, ep);
                     }
@@ -1014,7 +1040,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
             if (wl < cn) {
                 const int it = tbase + s_q[c0 + wl];
                 const int ilc = min(max(f.lcids[it], 0), f.nlc - 1);
-                const PointCtx c = point_ctx(f, r, pv, ilc);
+                const PointCtx c = ctx_of(ilc);
                 double ep;
                 s_ptc[wl] = (float)phase_of(f.times[it], c.t0v, p, inv_p, ep);
                 s_pk[wl] = (float)c.kk;
@@ -1078,7 +1104,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     for (int q = wl; q < n; q += kThreads) {
         const int it = tbase + s_q[q];
         const int ilc = f.nlc == 1 ? 0 : min(max(f.lcids[it], 0), f.nlc - 1);
-        const PointCtx c = point_ctx(f, r, pv, ilc);
+        const PointCtx c = ctx_of(ilc);
         double ep;
         const double tc = phase_of(f.times[it], c.t0v, p, inv_p, ep);
         const size_t row = ((size_t)pv * f.npb + c.ipb) * f.ng;
@@ -1342,8 +1368,8 @@ static int prepare_flux(const char* who, int precision, const rr_table* tab, con
     fa.ntiles = (npt + kTile - 1) / kTile;
     nblocks = (long long)fa.ntiles * npv;
     if (nblocks > 0x7fffffffLL) return set_error(RR_ERANGE, "%s: grid too large", who);
-    lds = flux_lds_bytes(mixed, tab->ng, fa.chunk_pts, fa.uniform_ns, fa.row_lds != 0, nlc > 1);
     fast = lc->sorted != 0;
+    lds = flux_lds_bytes(mixed, tab->ng, fa.chunk_pts, fa.uniform_ns, fa.row_lds != 0, nlc > 1);
     return RR_OK;
 }
 
