@@ -410,46 +410,51 @@ __device__ __forceinline__ void ts_store(Real* o, const Real* v, int n, bool vec
 // i + 1024 of every row, keeps their geometry in registers, and each step writes
 // one whole row.  The arithmetic per value is that of the tile form's cached path.
 constexpr int kTsRowThreads = 1024;
-constexpr int kTsRowVec = 2;      // time vectors per thread: npt <= 4 * 1024 * 2
 constexpr int kTsRowPc = 64;      // passbands per staged chunk
+// time vectors (16 bytes: 4 FP32 or 2 FP64 times) per thread; npt <= 8192 (FP32), 6144 (FP64)
+template <typename Real> constexpr int ts_row_vec() { return sizeof(Real) == 4 ? 2 : 3; }
+template <typename Real> constexpr int ts_row_max_npt() { return kTsRowThreads * ts_row_vec<Real>() * ts_tpl<Real>(); }
 
-static inline size_t ts_rows_lds(int ng) { return ((size_t)kTsRowPc * ng + 3 * kTsRowPc) * sizeof(float); }
+static inline size_t ts_rows_lds(int ng, size_t real) { return ((size_t)kTsRowPc * ng + 3 * kTsRowPc) * real; }
 
-__global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_kernel(TsArgs s, float* flux) {
+template <typename Real>
+__global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_kernel(TsArgs s, Real* flux) {
+    constexpr int TPL = ts_tpl<Real>();
+    constexpr int NV = ts_row_vec<Real>();
     extern __shared__ __align__(16) unsigned char lds[];
-    float* s_ldm = (float*)lds;                      // [kTsRowPc][ng]
-    float* s_afac = s_ldm + (size_t)kTsRowPc * s.ng;
-    float* s_kd = s_afac + kTsRowPc;
-    float* s_iist = s_kd + kTsRowPc;
-    typedef float f4 __attribute__((ext_vector_type(4)));
+    Real* s_ldm = (Real*)lds;                        // [kTsRowPc][ng]
+    Real* s_afac = s_ldm + (size_t)kTsRowPc * s.ng;
+    Real* s_kd = s_afac + kTsRowPc;
+    Real* s_iist = s_kd + kTsRowPc;
 
     const int pv = xcd_remap(blockIdx.x, gridDim.x);
-    const int npt = s.npt, nv = npt / 4;
+    const int npt = s.npt, nv = npt / TPL;
     const TsRec& r = s.rec[pv];
-    float* out = flux + (size_t)pv * s.npb * npt;
+    Real* out = flux + (size_t)pv * s.npb * npt;
+    Real vnan[TPL];
+#pragma unroll
+    for (int j = 0; j < TPL; ++j) vnan[j] = Real(NAN);
     if (!r.good) {
-        const f4 w = {NAN, NAN, NAN, NAN};
-        for (size_t i = threadIdx.x; i < (size_t)s.npb * nv; i += kTsRowThreads)
-            __builtin_nontemporal_store(w, (f4*)out + i);
+        for (size_t i = threadIdx.x; i < (size_t)s.npb * nv; i += kTsRowThreads) ts_store(out + i * TPL, vnan, TPL, true);
         return;
     }
     // the geometry of the thread's times, once
     const double ext = s.exptimes[0];
     const double km = r.kmean;
-    TsGeo<float> geo[kTsRowVec][4];
-    bool inw[kTsRowVec][4];
+    TsGeo<Real> geo[NV][TPL];
+    bool inw[NV][TPL];
 #pragma unroll
-    for (int u = 0; u < kTsRowVec; ++u) {
+    for (int u = 0; u < NV; ++u) {
         const int v = threadIdx.x + u * kTsRowThreads;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < TPL; ++j) {
             inw[u][j] = false;
-            geo[u][j].ra = geo[u][j].rb = geo[u][j].ap0 = geo[u][j].dadk = 0.0f;
+            geo[u][j].ra = geo[u][j].rb = geo[u][j].ap0 = geo[u][j].dadk = Real(0);
             geo[u][j].ig = 0;
             geo[u][j].inside = geo[u][j].gz = false;
             if (v < nv) {
 #pragma clang fp contract(off)
-                const double t = s.times[4 * v + j];
+                const double t = s.times[TPL * v + j];
                 const double t0v = s.t0[pv];
                 const double epoch = floor((t - t0v + 0.5 * r.p) * r.inv_p);
                 const double uu = t - t0v + 0.5 * r.p;
@@ -457,41 +462,43 @@ __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_kernel(TsArgs s, f
                 const double ep = epoch + ((rem < 0.0) ? -1.0 : ((rem >= r.p) ? 1.0 : 0.0));
                 const double tc = t - (t0v + ep * r.p);
                 inw[u][j] = (r.bt1 <= tc) && (tc <= r.bt4);
-                if (inw[u][j]) geo[u][j] = ts_geometry<float>(r, tc, ext, 1, 0, s.dg, s.ng);
+                if (inw[u][j]) geo[u][j] = ts_geometry<Real>(r, tc, ext, 1, 0, s.dg, s.ng);
             }
         }
     }
     for (int pb0 = 0; pb0 < s.npb; pb0 += kTsRowPc) {
         const int npc = min(kTsRowPc, s.npb - pb0);
         __syncthreads();  // the previous chunk's rows are no longer read
-        for (int i = threadIdx.x; i < npc * s.ng; i += kTsRowThreads)
-            s_ldm[i] = s.ldm32[((size_t)pv * s.npb + pb0) * s.ng + i];
+        for (int i = threadIdx.x; i < npc * s.ng; i += kTsRowThreads) {
+            const size_t src = ((size_t)pv * s.npb + pb0) * s.ng + i;
+            s_ldm[i] = sizeof(Real) == 4 ? (Real)s.ldm32[src] : (Real)s.ldm64[src];
+        }
         for (int q = threadIdx.x; q < npc; q += kTsRowThreads) {
             const double kq = s.k[(size_t)pv * s.npb + pb0 + q];
-            s_afac[q] = (float)(kq * kq / (km * km));
-            s_kd[q] = (float)(kq - km);
-            s_iist[q] = (float)(1.0 / s.istar[(size_t)pv * s.npb + pb0 + q]);
+            s_afac[q] = (Real)(kq * kq / (km * km));
+            s_kd[q] = (Real)(kq - km);
+            s_iist[q] = (Real)(1.0 / s.istar[(size_t)pv * s.npb + pb0 + q]);
         }
         __syncthreads();
 #pragma unroll 1
         for (int q = 0; q < npc; ++q) {
-            const float* l = s_ldm + (size_t)q * s.ng;
-            const float f = s_iist[q] * 1.0f;  // inv_ns = 1
-            const float afac = s_afac[q], kd = s_kd[q];
-            f4* o = (f4*)(out + (size_t)(pb0 + q) * npt);
+            const Real* l = s_ldm + (size_t)q * s.ng;
+            const Real f = s_iist[q] * Real(1);  // one sample: inv_ns = 1
+            const Real afac = s_afac[q], kd = s_kd[q];
+            Real* o = out + (size_t)(pb0 + q) * npt;
 #pragma unroll
-            for (int u = 0; u < kTsRowVec; ++u) {
+            for (int u = 0; u < NV; ++u) {
                 const int v = threadIdx.x + u * kTsRowThreads;
                 if (v < nv) {
-                    f4 w;
+                    Real w[TPL];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const TsGeo<float>& g = geo[u][j];
-                        const float ip = g.gz ? 0.0f : (g.rb * l[g.ig] + g.ra * l[g.ig + 1]);
-                        const float area = g.inside ? g.ap0 * afac : g.ap0 + kd * g.dadk;
-                        w[j] = inw[u][j] ? 1.0f - (ip * area) * f : 1.0f;
+                    for (int j = 0; j < TPL; ++j) {
+                        const TsGeo<Real>& g = geo[u][j];
+                        const Real ip = g.gz ? Real(0) : (g.rb * l[g.ig] + g.ra * l[g.ig + 1]);
+                        const Real area = g.inside ? g.ap0 * afac : g.ap0 + kd * g.dadk;
+                        w[j] = inw[u][j] ? Real(1) - (ip * area) * f : Real(1);
                     }
-                    __builtin_nontemporal_store(w, o + v);
+                    ts_store(o + (size_t)v * TPL, w, TPL, true);
                 }
             }
         }
@@ -735,11 +742,18 @@ int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
     if (nb > 0x7fffffffLL) return set_error(RR_ERANGE, "ts_evaluate: grid too large");
     const dim3 grid((unsigned)nb), block(kTsThreads);
     const int uns = lc->uniform_nsamples;
-    if (mixed && uns == 1 && npt % 4 == 0 && npt <= 4 * kTsRowThreads * kTsRowVec && ((uintptr_t)flux % 16) == 0 &&
-        ts_rows_lds(a.ng) <= 64 * 1024) {
-        hipLaunchKernelGGL(ts_flux_rows_kernel, dim3((unsigned)npv), dim3(kTsRowThreads), ts_rows_lds(a.ng), st, a,
-                           (float*)flux);
-        return check_launch("ts_flux_rows_kernel");
+    // one sample, rows of whole 16-byte vectors: the row form
+    if (uns == 1 && ((uintptr_t)flux % 16) == 0) {
+        if (mixed && npt % 4 == 0 && npt <= ts_row_max_npt<float>() && ts_rows_lds(a.ng, 4) <= 64 * 1024) {
+            hipLaunchKernelGGL(ts_flux_rows_kernel<float>, dim3((unsigned)npv), dim3(kTsRowThreads),
+                               ts_rows_lds(a.ng, 4), st, a, (float*)flux);
+            return check_launch("ts_flux_rows_kernel");
+        }
+        if (!mixed && npt % 2 == 0 && npt <= ts_row_max_npt<double>() && ts_rows_lds(a.ng, 8) <= 64 * 1024) {
+            hipLaunchKernelGGL(ts_flux_rows_kernel<double>, dim3((unsigned)npv), dim3(kTsRowThreads),
+                               ts_rows_lds(a.ng, 8), st, a, (double*)flux);
+            return check_launch("ts_flux_rows_kernel");
+        }
     }
     if (mixed) {
         const int PC = ts_pb_chunk(true, cached);

@@ -47,16 +47,17 @@ def test_ts_tabulated_profiles(gpu, oracle, prec, tol):
 
 
 @pytest.mark.parametrize('npt', [1501, 1500])
-def test_ts_mixed_tile_and_row_forms(gpu, oracle, npt):
-    # npt % 4 != 0 takes the tile form (one workgroup per (pv, 1024 times)), a
+@pytest.mark.parametrize('prec,tol', [('fp64', FP64_TOL), ('mixed', MIXED_TOL)])
+def test_ts_tile_and_row_forms(gpu, oracle, npt, prec, tol):
+    # an odd npt takes the tile form (one workgroup per (pv, tile of times)), a
     # multiple of 4 the row form (one workgroup per pv writing its rows in order);
     # both must give the oracle's flux, and the NaN row of an invalid pv
-    say(f'case: TS mixed, 3 pv x 40 pb x {npt} t, one invalid pv')
+    say(f'case: TS {prec}, 3 pv x 40 pb x {npt} t, one invalid pv')
     wl, ze, mu = ts_case(oracle, npv=3, npt=npt, npb=40, seed=5)
     wl.a = wl.a.copy()
     wl.a[1] = 0.5  # a <= 1: an invalid pv, a NaN row
     fo = oracle_ts(oracle, wl, ze)
-    tm = TSModel(TabulatedLDModel(wl.ldp, wl.istar), nzin=64, nzlimb=64, precision='mixed')
+    tm = TSModel(TabulatedLDModel(wl.ldp, wl.istar), nzin=64, nzlimb=64, precision=prec)
     tm.set_data(wl.times)
     fg = np.asarray(tm.evaluate(wl.k, None, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w), float)
     assert fg.shape == fo.shape == (3, 40, npt)
@@ -64,7 +65,7 @@ def test_ts_mixed_tile_and_row_forms(gpu, oracle, npt):
     err = np.max(np.abs(fg[[0, 2]] - fo[[0, 2]]))
     say(f'  max |dF| = {err:.3e}, in-transit points {(fo[[0, 2]] < 1).sum()}')
     assert (fo[[0, 2]] < 1).sum() > 1000
-    assert err <= MIXED_TOL
+    assert err <= tol
 
 
 @pytest.mark.parametrize('prec,tol', [('fp64', FP64_TOL), ('mixed', MIXED_TOL)])
