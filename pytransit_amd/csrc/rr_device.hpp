@@ -302,6 +302,67 @@ __device__ __forceinline__ double ld_law(int law, double mu, const double* c, in
     }
 }
 
+// ---- Laws that are linear in their coefficients -------------------------------------
+//
+// I(mu) = sum_i coef_i(c) B_i(mu) for every named law but power-2, power-2-pm and
+// tabulated profiles: the constant and the coefficient terms of ldmodels.py:22-113.
+// Then every linear functional of the profile (its limb-darkening means, its disk
+// integral) of any passband is the same combination of the functional of the
+// basis, which a pv computes once for all its passbands.
+constexpr int kMaxBasis = 6;
+
+// Number of basis functions of the law, 0 when it is not linear in its coefficients.
+__host__ __device__ inline int ld_basis_count(int law, int nc) {
+    switch (law) {
+        case RR_LD_UNIFORM: return 1;
+        case RR_LD_LINEAR: return 2;
+        case RR_LD_QUADRATIC:
+        case RR_LD_QUADRATIC_TRI:
+        case RR_LD_SQUARE_ROOT:
+        case RR_LD_LOGARITHMIC:
+        case RR_LD_EXPONENTIAL: return 3;
+        case RR_LD_NONLINEAR: return 5;
+        case RR_LD_GENERAL: return nc >= 1 && nc <= kMaxBasis ? nc : 0;
+        default: return 0;
+    }
+}
+
+// B_i(mu), in the form of the matching term of ld_law.
+__device__ __forceinline__ double ld_basis(int law, int i, double mu) {
+    if (law == RR_LD_GENERAL) {
+        double mp = mu;  // mu^(i+1) by repeated products
+        for (int j = 0; j < i; ++j) mp *= mu;
+        return 1.0 - mp;
+    }
+    if (i == 0) return 1.0;
+    switch (law) {
+        case RR_LD_LINEAR: return 1.0 - mu;
+        case RR_LD_QUADRATIC:
+        case RR_LD_QUADRATIC_TRI: return i == 1 ? 1.0 - mu : (1.0 - mu) * (1.0 - mu);
+        case RR_LD_NONLINEAR: {
+            const double sm = sqrt(mu);
+            return i == 1 ? 1.0 - sm : (i == 2 ? 1.0 - mu : (i == 3 ? 1.0 - mu * sm : 1.0 - mu * mu));
+        }
+        case RR_LD_SQUARE_ROOT: return i == 1 ? 1.0 - mu : 1.0 - sqrt(mu);
+        case RR_LD_LOGARITHMIC: return i == 1 ? 1.0 - mu : mu * log(mu);
+        case RR_LD_EXPONENTIAL: return i == 1 ? 1.0 - mu : 1.0 / (1.0 - exp(mu));
+        default: return NAN;
+    }
+}
+
+// coef_i of the coefficients c[0..nc).
+__device__ __forceinline__ double ld_basis_coef(int law, int i, const double* c) {
+    switch (law) {
+        case RR_LD_GENERAL: return c[i];
+        case RR_LD_QUADRATIC_TRI: {
+            if (i == 0) return 1.0;
+            const double a = sqrt(c[0]), b = 2.0 * c[1];
+            return i == 1 ? -(a * b) : -(a * (1.0 - b));
+        }
+        default: return i == 0 ? 1.0 : -c[i - 1];
+    }
+}
+
 // Analytic disk integral where the reference has one (ldmodels.py:27-123); false
 // for the laws that use the 200-node trapezoid (rrmodel.py:220-227, Q8).
 __device__ __forceinline__ bool ld_integral(int law, const double* c, double& out) {
@@ -393,6 +454,30 @@ __device__ __forceinline__ double ld_trapz_wave(int law, const double* c, int nc
             const double mu = linspace_at(1.0, 0.0, 200, m);
             z = sqrt(1.0 - mu * mu);
             y = z * ld_law(law, mu, c, nc);
+        }
+        double zl = __shfl_up(z, 1, 64), yl = __shfl_up(y, 1, 64);
+        if (lane == 0) {
+            zl = carry_z;
+            yl = carry_y;
+        }
+        if (m >= 1 && m < 200) part += (z - zl) * (y + yl) / 2.0;
+        carry_z = __shfl(z, 63, 64);
+        carry_y = __shfl(y, 63, 64);
+    }
+    return 2.0 * kPi * wave_sum(part);
+}
+
+// 2*pi*trapz(z*B_i(mu), z) on mu = linspace(1, 0, 200), as ld_trapz_wave for a law.
+__device__ __forceinline__ double ld_basis_trapz_wave(int law, int i) {
+    const int lane = threadIdx.x & 63;
+    double part = 0.0, carry_z = 0.0, carry_y = 0.0;
+    for (int base = 0; base < 200; base += 64) {
+        const int m = base + lane;
+        double z = 0.0, y = 0.0;
+        if (m < 200) {
+            const double mu = linspace_at(1.0, 0.0, 200, m);
+            z = sqrt(1.0 - mu * mu);
+            y = z * ld_basis(law, i, mu);
         }
         double zl = __shfl_up(z, 1, 64), yl = __shfl_up(y, 1, 64);
         if (lane == 0) {

@@ -158,7 +158,7 @@ struct SetupArgs {
 __device__ __forceinline__ int lc0_pb(const int32_t* pbids, int npb) { return min(max(pbids[0], 0), npb - 1); }
 __device__ __forceinline__ int lc0_ep(const int32_t* epids, int nep) { return min(max(epids[0], 0), nep - 1); }
 
-__device__ void setup_orbit_role(const SetupArgs& s, int blk, double* sm) {
+__device__ __forceinline__ void setup_orbit_role(const SetupArgs& s, int blk, double* sm) {
     const int pv = blk * kOrbitGroup + threadIdx.x / kOrbitGroup;
     const int j = threadIdx.x % kOrbitGroup;
     const bool inrange = pv < s.npv;
@@ -186,7 +186,23 @@ __device__ void setup_orbit_role(const SetupArgs& s, int blk, double* sm) {
 constexpr int kLdGroup = 8;
 constexpr int kLdLds = 1024;  // doubles of profile storage per block (>= kMaxNz)
 
-__host__ __device__ inline int ld_group_size(int nz) {
+// A law linear in its coefficients with more passbands than basis functions is
+// evaluated through its basis (ld_means_basis): one block takes up to
+// kLdGroupBasis passbands, whose basis profiles [nb][nz], coefficients [gs][nb]
+// and integrals [gs] share the kLdLds doubles.
+constexpr int kLdGroupBasis = 64;
+
+__host__ __device__ inline int ld_basis_group(int nz, int law, int nldc, int npb) {
+    const int nb = ld_basis_count(law, nldc);
+    if (nb == 0 || npb <= nb) return 0;
+    const int room = kLdLds - nb * nz;
+    const int gs = room > 0 ? room / (nb + 1) : 0;
+    return gs < kLdGroup ? 0 : (gs < kLdGroupBasis ? gs : kLdGroupBasis);
+}
+
+__host__ __device__ inline int ld_group_size(int nz, int law, int nldc, int npb) {
+    const int gb = ld_basis_group(nz, law, nldc, npb);
+    if (gb > 0) return gb;
     const int g = kLdLds / (nz > 0 ? nz : 1);
     return g < kLdGroup ? (g < 1 ? 1 : g) : kLdGroup;
 }
@@ -369,12 +385,127 @@ __device__ __forceinline__ void ld_means_vec(const SetupArgs& s, size_t j, int i
     }
 }
 
-__device__ void setup_ld_role(const SetupArgs& s, int pv, int grp, double* sm) {
+// The limb-darkening role through the law's basis (ld_basis_count > 0): basis
+// profiles at the mu nodes, the basis means M_b(g) from the table rows (lerped in k
+// once per value) or the exact 2-D weights, then for each of the gn passbands
+// ldm(g) = sum_b coef_b M_b(g) and I* = its analytic integral or
+// sum_b coef_b trapz(B_b).  The same sums as the per-passband form up to rounding.
+__device__ __forceinline__ void ld_basis_role(const SetupArgs& s, int pv, int pb0, int gn, int gs, size_t j0, bool in_table, int ik,
+                              double ak, double k0, double* sm) {
+    const int lane = threadIdx.x;
+    const int nz = s.nz, ng = s.ng, law = s.law;
+    const int nb = ld_basis_count(law, s.nldc);
+    double* s_B = sm;                    // [nb][nz]
+    double* s_coef = sm + nb * nz;       // [gs][nb]
+    double* s_istq = s_coef + gs * nb;   // [gs]
+    for (int i = lane; i < nb * nz; i += 64) {
+        const int b = i / nz;
+        s_B[i] = ld_basis(law, b, s.mu[i - b * nz]);
+    }
+    for (int i = lane; i < gn * nb; i += 64) {
+        const int q = i / nb;
+        s_coef[i] = ld_basis_coef(law, i - q * nb, s.ldc + (j0 + q) * s.nldc);
+    }
+    // disk integrals: analytic per passband, or the basis trapezoids (wave-wide)
+    const bool analytic = law == RR_LD_UNIFORM || law == RR_LD_LINEAR || law == RR_LD_QUADRATIC ||
+                          law == RR_LD_QUADRATIC_TRI;
+    double tb[kMaxBasis];
+#pragma unroll
+    for (int b = 0; b < kMaxBasis; ++b) tb[b] = (!analytic && b < nb) ? ld_basis_trapz_wave(law, b) : 0.0;
+    __syncthreads();
+    for (int q = lane; q < gn; q += 64) {
+        double ist;
+        if (analytic) {
+            ld_integral(law, s.ldc + (j0 + q) * s.nldc, ist);
+        } else {
+            ist = 0.0;
+#pragma unroll
+            for (int b = 0; b < kMaxBasis; ++b)
+                if (b < nb) ist += s_coef[q * nb + b] * tb[b];
+        }
+        s_istq[q] = ist;
+        s.istar[j0 + q] = ist;
+    }
+    if (lane == 0 && pb0 == 0)
+        s.rec[pv].ldok = isnan(ld_law(law, s.mu[0], s.ldc + j0 * s.nldc, s.nldc)) ? 0 : 1;
+    __syncthreads();
+    {
+        const int q0 = lc0_pb(s.pbids, s.npb) - pb0;
+        if (lane == 0 && q0 >= 0 && q0 < gn) s.rec[pv].ist_lc0 = s_istq[q0];
+    }
+    // basis means, lanes over g
+    const bool mixed = s.ldm32 != nullptr;
+    for (int ig = lane; ig < ng; ig += 64) {
+        double m[kMaxBasis];
+#pragma unroll
+        for (int b = 0; b < kMaxBasis; ++b) m[b] = 0.0;
+        if (in_table) {
+            const size_t o0 = (size_t)ik * nz * ng + ig, o1 = o0 + (size_t)nz * ng;
+            constexpr int kB = 8;
+            for (int iz0 = 0; iz0 < nz; iz0 += kB) {
+                double w0[kB], w1[kB];
+#pragma unroll
+                for (int u = 0; u < kB; ++u) {
+                    const size_t r = (size_t)min(iz0 + u, nz - 1) * ng;
+                    if (mixed && s.W32) {
+                        w0[u] = (double)s.W32[o0 + r];
+                        w1[u] = (double)s.W32[o1 + r];
+                    } else {
+                        w0[u] = s.W[o0 + r];
+                        w1[u] = s.W[o1 + r];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kB; ++u) {
+                    if (iz0 + u < nz) {
+                        const double wk = (1.0 - ak) * w0[u] + ak * w1[u];
+#pragma unroll
+                        for (int b = 0; b < kMaxBasis; ++b)
+                            if (b < nb) m[b] += wk * s_B[b * nz + iz0 + u];
+                    }
+                }
+            }
+        } else {
+            // common.py:152-185: normalised annulus areas
+            const double g = linspace_at(0.0, 1.0 - 1e-7, ng, ig);
+            const double bb = g * (1.0 + k0);
+            double a0 = cci_area(s.ze[0], k0, bb), sum = a0;
+            for (int iz = 1; iz < nz; ++iz) {
+                const double a1 = cci_area(s.ze[iz], k0, bb);
+                sum += a1 - a0;
+                a0 = a1;
+            }
+            double b0 = cci_area(s.ze[0], k0, bb);
+#pragma unroll
+            for (int b = 0; b < kMaxBasis; ++b) m[b] = b < nb ? (b0 / sum) * s_B[b * nz] : 0.0;
+            for (int iz = 1; iz < nz; ++iz) {
+                const double b1 = cci_area(s.ze[iz], k0, bb);
+                const double wv = (b1 - b0) / sum;
+#pragma unroll
+                for (int b = 0; b < kMaxBasis; ++b)
+                    if (b < nb) m[b] += wv * s_B[b * nz + iz];
+                b0 = b1;
+            }
+        }
+        // every passband of the group from the basis means
+        for (int q = 0; q < gn; ++q) {
+            double v = 0.0;
+#pragma unroll
+            for (int b = 0; b < kMaxBasis; ++b)
+                if (b < nb) v += s_coef[q * nb + b] * m[b];
+            const size_t o = (j0 + q) * ng + ig;
+            if (mixed) s.ldm32[o] = (float)v;
+            else s.ldm64[o] = v;
+        }
+    }
+}
+
+__device__ __forceinline__ void setup_ld_role(const SetupArgs& s, int pv, int grp, double* sm) {
     double* s_l = sm;                // kLdLds
     double* s_ist = sm + kLdLds;     // kLdGroup
     const int lane = threadIdx.x;
     const int nz = s.nz, ng = s.ng;
-    const int gs = ld_group_size(nz);
+    const int gs = ld_group_size(nz, s.law, s.nldc, s.npb);
     const int pb0 = grp * gs;
     const int gn = min(gs, s.npb - pb0);
     const double k0 = s.k[(size_t)pv * s.kcols];
@@ -387,6 +518,10 @@ __device__ void setup_ld_role(const SetupArgs& s, int pv, int grp, double* sm) {
         ak = (k0 - s.kmin - ik * s.dk) / s.dk;
     }
     const size_t j0 = (size_t)pv * s.npb + pb0;
+    if (ld_basis_group(nz, s.law, s.nldc, s.npb) > 0) {
+        ld_basis_role(s, pv, pb0, gn, gs, j0, in_table, ik, ak, k0, sm);
+        return;
+    }
     // profiles at the mu nodes, lanes over (passband, node), and their disk integrals
     const bool tab = s.law == RR_LD_TABULATED;
     for (int i = lane; i < gn * nz; i += 64) {
@@ -435,7 +570,8 @@ __device__ __forceinline__ void setup_body(const SetupArgs& s, int blk) {
         setup_orbit_role(s, blk, sm);
     } else {
         const int b = blk - s.n_orbit_blocks;
-        const int ngrp = (s.npb + ld_group_size(s.nz) - 1) / ld_group_size(s.nz);
+        const int gsz = ld_group_size(s.nz, s.law, s.nldc, s.npb);
+        const int ngrp = (s.npb + gsz - 1) / gsz;
         const int pv = b / ngrp;
         setup_ld_role(s, pv, b - pv * ngrp, sm);
     }
@@ -1333,7 +1469,7 @@ static int prepare_setup(const char* who, int precision, const rr_table* tab, co
     sa.npv = npv; sa.npb = npb; sa.kcols = pv->kcols; sa.nk = tab->nk; sa.ng = tab->ng; sa.nz = tab->nz;
     sa.kmin = tab->kmin; sa.kmax = tab->kmax; sa.dk = tab->dk;
     sa.n_orbit_blocks = (npv + kOrbitGroup - 1) / kOrbitGroup;
-    const int gs = ld_group_size(tab->nz);
+    const int gs = ld_group_size(tab->nz, ld->law, ld->nldc, npb);
     const long long nb = (long long)sa.n_orbit_blocks + (long long)npv * ((npb + gs - 1) / gs);
     if (nb > 0x7fffffffLL) return set_error(RR_ERANGE, "%s: setup grid too large", who);
     nblocks = (int)nb;
