@@ -207,6 +207,9 @@ class RRServer:
         flux outputs (one arena, allocated at construction).
     slots : batches in flight at once (one HIP stream each, default priority).
     max_batch : requests per batch (at most ``rr_max_batch()`` of the library).
+    max_batch_points : model points per batch (None: no limit).  A request above
+        it runs alone; smaller ones are batched up to it, so they do not wait for
+        a large request that happens to share their launch.
     precision : 'mixed' or 'fp64', as RoadRunnerModel.
     on_done : called with each finished request; with ``auto_release`` its flux
         memory is returned right after the call.
@@ -217,8 +220,9 @@ class RRServer:
     def __init__(self, hbm_budget: int = 16 << 30, slots: int = 4, max_batch: int = 16, precision: str = 'mixed',
                  device=None, on_done: Optional[Callable[[Request], None]] = None, auto_release: bool = False,
                  max_skips: int = 8, staging_bytes: int = 64 << 20, klims=(0.005, 0.5), nk=256, nzin=20,
-                 nzlimb=20, zcut=0.7, ng=100, executor=None):
+                 nzlimb=20, zcut=0.7, ng=100, executor=None, max_batch_points: Optional[int] = None):
         self.precision = precision
+        self.max_batch_points = max_batch_points
         self._prec = _PRECISIONS[precision]
         self.max_skips = max_skips
         self.on_done, self.auto_release = on_done, auto_release
@@ -350,19 +354,25 @@ class RRServer:
     def _admit(self) -> List[Request]:
         """Up to max_batch waiting requests in arrival order.  A request that does
         not fit is passed over; once one has been passed over max_skips times, no
-        later request is admitted until it fits."""
+        later request is admitted until it fits.  A request that would take the
+        batch above max_batch_points waits, without counting as passed over, for
+        a later batch, which it heads."""
         batch, rest, passed = [], deque(), []
-        staged = 0
+        staged = points = 0
         blocked = False
         while self._queue:
             r = self._queue.popleft()
             if blocked or len(batch) >= self.max_batch:
                 rest.append(r)
                 continue
+            if batch and self.max_batch_points is not None and points + r.points > self.max_batch_points:
+                rest.append(r)
+                continue
             size = self._exec.staging_need(r)
             if staged + size <= self._exec.staging_bytes and self._place(r):
                 batch.append(r)
                 staged += size
+                points += r.points
                 for q in passed:
                     q.skips += 1
                 passed = []

@@ -218,3 +218,26 @@ def test_request_validation():
     r = srv.submit(ds, np.full((4, 3), 0.1), [0.4, 0.2], np.zeros(4), np.full(4, 3.0), 10.0, 1.5,
                    ld='quadratic')
     assert r.host['ldc'].shape == (4, 3, 2) and r.host['a'].shape == (4,)
+
+
+def test_batch_points_cap_runs_large_requests_alone():
+    ref = [None]
+    ex = FakeExec(ref)
+    srv = RRServer(hbm_budget=1 << 30, slots=2, max_batch=8, precision='mixed', executor=ex, max_skips=3,
+                   auto_release=True, max_batch_points=50_000)
+    ref[0] = srv
+    ds = dataset(npt=1000)
+    small1 = submit(srv, ds, 10)     # 10 000 points
+    big = submit(srv, ds, 200)       # 200 000 points: above the cap, alone
+    small2 = submit(srv, ds, 20)     # 20 000 points
+    small3 = submit(srv, ds, 30)     # 30 000 points: would pass the cap with small1 + small2
+    srv.step()
+    first, second = [[r.rid for r in reqs] for _, reqs, _ in ex.launches[:2]]
+    # the small requests before the cap share a batch; the large one heads the next
+    assert first == [small1.rid, small2.rid]
+    assert second == [big.rid]
+    drain(srv, ex)
+    third = [r.rid for r in ex.launches[2][1]]
+    assert third == [small3.rid]
+    assert all(r.done for r in (small1, big, small2, small3))
+    assert sum(r.points for r in (small1, small2)) <= 50_000
