@@ -227,6 +227,37 @@ def test_bjd_segments_many_passbands(gpu, oracle):
         assert err <= tol
 
 
+@pytest.mark.parametrize('law,npb,nldc', [('quadratic', 9, 2), ('nonlinear', 9, 4), ('square_root', 7, 2),
+                                          ('general', 6, 3), ('linear', 5, 1), ('quadratic-tri', 6, 2)])
+def test_linear_laws_many_passbands_basis(gpu, oracle, law, npb, nldc):
+    # more passbands than basis functions: the setup evaluates the limb-darkening
+    # means and disk integrals through the law's basis (one table pass per pv);
+    # pv 0 has k off the table (exact 2-D weights), pv 1 a NaN coefficient in
+    # passband 0 (a NaN row), pv 2 a NaN coefficient in passband 3
+    say(f'case: {law}, {npb} contiguous passband light curves, nss=3, basis limb darkening')
+    wl = workloads.rr_bjd(npv=20, npt=3000, seed=11)
+    wl.law = law
+    wl.nsamples = 3
+    lcids = (np.arange(wl.times.size) * npb) // wl.times.size
+    rng = np.random.default_rng(11)
+    wl.k = wl.k + 0.002 * np.sin(np.arange(npb) / 3.0)[None, :]
+    wl.k[0, 0] = 0.52
+    base = {1: [0.5], 2: [0.4, 0.2], 3: [0.5, -0.1, 0.05], 4: [0.56, -0.08, 0.67, -0.33]}[nldc]
+    wl.ldc = np.asarray(base)[None, None, :] + rng.uniform(-0.03, 0.03, (20, npb, nldc))
+    wl.ldc[1, 0, 0] = np.nan
+    wl.ldc[2, 3, -1] = np.nan
+    pbids = np.arange(npb)
+    fo = oracle_flux(oracle, wl, law, lcids=lcids, pbids=pbids)
+    assert (fo < 1).sum() > 500
+    assert np.isnan(fo[1]).all()
+    for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
+        fg = gpu_flux(wl, prec, lcids=lcids, pbids=pbids)
+        assert np.array_equal(np.isnan(fo), np.isnan(fg)), prec
+        err = np.nanmax(np.abs(fg - fo))
+        say(f'  {prec}: max |dF| = {err:.3e}')
+        assert err <= tol
+
+
 def test_k_outside_table_uses_exact_weights(gpu, oracle):
     say('case: k outside klims -> exact 2-D weights')
     wl = workloads.rr_small(npv=8)
