@@ -862,6 +862,67 @@ __device__ __forceinline__ int queue_partial(bool part, int i, T z, int pq, uint
     return pq + __popcll(mk);
 }
 
+// In-window index ranges of a tile that lies in one light curve sorted in time.
+// For the window of epoch e, tc = t - (t0 + e p) is monotone in the point index, so
+// the window is the index range [first tc >= lo, first tc > hi).  The two edges of
+// each of the (at most two) epochs the tile spans are found by a 64-way search:
+// one time per lane at 16-point spacing brackets every edge, then 16 lanes per edge
+// test the 15 points of its bracket.  Every point gets the classification
+// model_full.py:89-91 gives it: near a window, a point's own epoch is the window's
+// (windows narrower than half a period).  Returns false, leaving the run
+// classification to decide, when the tile spans more than two epochs or the window
+// is that wide.
+__device__ __forceinline__ bool window_edges(const double* times, int tbase, int cnt, const PointCtx& c, double p,
+                                             double ea, double eb, int wl, int (&edge)[4]) {
+    if (eb - ea > 1.0 || !(c.hi - c.lo < 0.5 * p)) return false;
+    const int nedge = ea == eb ? 2 : 4;
+    const double base0 = epoch_base(c.t0v, ea, p), base1 = epoch_base(c.t0v, eb, p);
+    // edge k: k even, first tc >= lo; k odd, first tc > hi; epoch ea for k < 2
+    auto pred = [&](int k, double t) -> bool {
+        const double tc = t - (k < 2 ? base0 : base1);
+        return (k & 1) ? (tc > c.hi) : (tc >= c.lo);
+    };
+    const int s = 16 * wl;
+    const double ts = s < cnt ? times[tbase + s] : 0.0;
+    int lo_i[4], hi_i[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        edge[k] = cnt;
+        lo_i[k] = -1;
+        hi_i[k] = cnt;
+        if (k < nedge) {
+            const unsigned long long mk = __ballot(s >= cnt || pred(k, ts));
+            const int fl = mk ? __ffsll((long long)mk) - 1 : 64;  // first sample with the edge passed
+            if (fl == 0) {
+                edge[k] = 0;
+            } else {
+                lo_i[k] = 16 * (fl - 1);   // edge in (lo_i, hi_i]
+                hi_i[k] = min(16 * fl, cnt);
+            }
+        }
+    }
+    // 16 lanes per edge: candidates lo_i + 1 .. lo_i + 15
+    const int k2 = wl >> 4, j = wl & 15;
+    int lo = lo_i[0], hi = hi_i[0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        lo = k2 == k ? lo_i[k] : lo;
+        hi = k2 == k ? hi_i[k] : hi;
+    }
+    const int idx = lo + j;
+    const bool active = lo >= 0 && j >= 1 && idx < hi;
+    const double tt = active ? times[tbase + idx] : 0.0;
+    const unsigned long long m2 = __ballot(active && pred(k2, tt));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (lo_i[k] >= 0) {
+            const unsigned int g = (unsigned int)((m2 >> (16 * k)) & 0xffffull);
+            edge[k] = g ? lo_i[k] + __ffs(g) - 1 : hi_i[k];
+        }
+    }
+    return true;
+}
+
 template <bool kMixed, bool kFast>
 __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     using Real = typename std::conditional<kMixed, float, double>::type;
@@ -914,6 +975,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
         return lc_lds ? s_lc[min(max(ilc - la0, 0), kTileLcs - 1)] : point_ctx(f, r, pv, ilc);
     };
     int count = 0;
+    bool ranged = false;  // the queue came from window_edges
     auto push = [&](bool pred, int it) {
         const unsigned long long mk = __ballot(pred);
         if (pred) s_q[count + __popcll(mk & ((1ull << wl) - 1ull))] = (uint16_t)(it - tbase);
@@ -941,11 +1003,26 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                     wave_fill<Real>(out, tbase, last + 1 - tbase, Real(1), wl);
                     return;
                 }
+                // the window's index ranges by search: ones around them, the
+                // ranges into the queue, no run classification
+                const int cnt = last + 1 - tbase;
+                int edge[4];
+                if (window_edges(f.times, tbase, cnt, c, p, ea, eb, wl, edge)) {
+                    const int r0 = edge[0], r1 = edge[1];
+                    const int r2 = ea == eb ? r1 : edge[2], r3 = ea == eb ? r1 : edge[3];
+                    wave_fill<Real>(out, tbase, r0, Real(1), wl);
+                    wave_fill<Real>(out, tbase + r1, r2 - r1, Real(1), wl);
+                    wave_fill<Real>(out, tbase + r3, cnt - r3, Real(1), wl);
+                    const int n0 = r1 - r0;
+                    count = n0 + (r3 - r2);
+                    for (int q = wl; q < count; q += kThreads) s_q[q] = (uint16_t)(q < n0 ? r0 + q : r2 + (q - n0));
+                    ranged = true;
+                }
             }
         }
         const PointCtx c1 = f.nlc == 1 ? point_ctx0(f, r) : ctx_of(la0);
 #pragma unroll
-        for (int m = 0; m < kRuns; ++m) {
+        for (int m = 0; m < (ranged ? 0 : kRuns); ++m) {
             bool all_out = true, one_lc = true, one_ep = false;
             double base = 0.0;
             PointCtx c = c1;

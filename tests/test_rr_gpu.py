@@ -258,6 +258,29 @@ def test_linear_laws_many_passbands_basis(gpu, oracle, law, npb, nldc):
         assert err <= tol
 
 
+def test_short_periods_window_search(gpu, oracle):
+    # periods of 0.45-1.6 d against 1.42-d tiles (1024 times at 2-min cadence):
+    # tiles spanning one, two (window ranges found by search) and three or more
+    # epochs (run classification), windows across tile edges, a 30-min exposure
+    say('case: 24 pv, periods 0.45-1.6 d, 6000 times at 2-min cadence, nss=7')
+    rng = np.random.default_rng(31)
+    npv = 24
+    wl = workloads.rr_bjd(npv=npv, npt=6000, seed=31)
+    wl.nsamples = 7
+    wl.p = rng.uniform(0.45, 1.6, npv)
+    wl.a = workloads.as_from_rhop(rng.uniform(1.0, 3.0, npv), wl.p)
+    wl.i = workloads.i_from_ba(rng.uniform(0.0, 0.9, npv), wl.a)
+    wl.t0 = (wl.times[0] + rng.uniform(0.0, 1.0, npv))[:, None]
+    fo = oracle_flux(oracle, wl, 'power-2')
+    assert (fo < 1).sum() > 5000
+    for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
+        fg = gpu_flux(wl, prec)
+        assert np.array_equal(np.isnan(fo), np.isnan(fg)), prec
+        err = np.nanmax(np.abs(fg - fo))
+        say(f'  {prec}: max |dF| = {err:.3e}')
+        assert err <= tol
+
+
 def test_k_outside_table_uses_exact_weights(gpu, oracle):
     say('case: k outside klims -> exact 2-D weights')
     wl = workloads.rr_small(npv=8)
