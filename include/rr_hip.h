@@ -74,11 +74,19 @@ typedef struct rr_ld {
     const double* istar;
 } rr_ld;
 
+/* Layout bits of rr_lcdata.sorted.  RR_SORTED_RUNS: lcids are non-decreasing and the
+ * times of each light curve are non-decreasing (enables run skipping).  With it,
+ * RR_SORTED_TIMES: the times are non-decreasing over the whole array, and
+ * RR_ONE_WINDOW: every light curve has the same epoch index and exposure time, so a
+ * pv has one transit window for all of them.  0 is always valid. */
+#define RR_SORTED_RUNS 1
+#define RR_SORTED_TIMES 2
+#define RR_ONE_WINDOW 4
+
 /* Light-curve data set by set_data() (transitmodel.py:56-125), device-resident. */
 typedef struct rr_lcdata {
     int32_t npt, nlc, npb, nep;
-    int32_t sorted;           /* 1 if lcids are non-decreasing and the times of each light
-                                 curve are non-decreasing (enables run skipping)          */
+    int32_t sorted;           /* layout bits, RR_SORTED_*: see below                  */
     int32_t uniform_nsamples; /* the common nsamples of all light curves, or 0         */
     const double* times;      /* [npt]                                               */
     const int32_t* lcids;     /* [npt], values in [0, nlc)                           */

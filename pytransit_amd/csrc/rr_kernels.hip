@@ -635,6 +635,7 @@ struct FluxArgs {
     int uniform_ns;                   // the common nsamples of all light curves, or 0
     int chunk_pts;                    // points per phase-B chunk (uniform_ns > 0)
     int row_lds;                      // stage the limb-darkening row in LDS (one light curve)
+    int one_window;                   // several light curves, times sorted across them, one window
     double dg;
 };
 
@@ -958,10 +959,12 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     const double p = r.p, inv_p = r.inv_p;
     // sorted light curves: the tile's points lie in light curves la0..lb0
     // (lcids are non-decreasing); up to kTileLcs of them are staged in LDS
+    // (staged once the tile is known to hold a window)
     PointCtx* s_lc = (PointCtx*)s_tc;
     int la0 = 0;
     bool lc_lds = false;
-    if (kFast && f.nlc > 1) {
+    auto stage_lcs = [&]() {
+        if (!(kFast && f.nlc > 1)) return;
         la0 = min(max(f.lcids[tbase], 0), f.nlc - 1);
         const int lb0 = min(max(f.lcids[min(tbase + kTile, f.npt) - 1], 0), f.nlc - 1);
         lc_lds = lb0 != la0 && lb0 - la0 < kTileLcs;
@@ -969,7 +972,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
             if (wl <= lb0 - la0) s_lc[wl] = point_ctx(f, r, pv, la0 + wl);
             __syncthreads();
         }
-    }
+    };
     // (the clamp keeps a caller that flags unsorted data as sorted inside the table)
     auto ctx_of = [&](int ilc) -> PointCtx {
         return lc_lds ? s_lc[min(max(ilc - la0, 0), kTileLcs - 1)] : point_ctx(f, r, pv, ilc);
@@ -990,10 +993,12 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
         // of a transit window and are filled with ones at once.
         {
             const int last = min(tbase + kTile, f.npt) - 1;
-            const int la = f.nlc == 1 ? 0 : min(max(f.lcids[tbase], 0), f.nlc - 1);
-            const int lb = f.nlc == 1 ? 0 : min(max(f.lcids[last], 0), f.nlc - 1);
+            const int la = f.nlc == 1 || f.one_window ? 0 : min(max(f.lcids[tbase], 0), f.nlc - 1);
+            const int lb = f.nlc == 1 || f.one_window ? 0 : min(max(f.lcids[last], 0), f.nlc - 1);
             if (la == lb) {
-                const PointCtx c = f.nlc == 1 ? point_ctx0(f, r) : ctx_of(la);
+                // the window of the tile's light curve, or the one window all
+                // light curves share (times then sorted across light curves too)
+                const PointCtx c = f.nlc == 1 || f.one_window ? point_ctx0(f, r) : point_ctx(f, r, pv, la);
                 double ea, eb;
                 const double tca = phase_of(f.times[tbase], c.t0v, p, inv_p, ea);
                 const double tcb = phase_of(f.times[last], c.t0v, p, inv_p, eb);
@@ -1020,6 +1025,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                 }
             }
         }
+        stage_lcs();
         const PointCtx c1 = f.nlc == 1 ? point_ctx0(f, r) : ctx_of(la0);
 #pragma unroll
         for (int m = 0; m < (ranged ? 0 : kRuns); ++m) {
@@ -1581,7 +1587,9 @@ static int prepare_flux(const char* who, int precision, const rr_table* tab, con
     fa.ntiles = (npt + kTile - 1) / kTile;
     nblocks = (long long)fa.ntiles * npv;
     if (nblocks > 0x7fffffffLL) return set_error(RR_ERANGE, "%s: grid too large", who);
-    fast = lc->sorted != 0;
+    fast = (lc->sorted & RR_SORTED_RUNS) != 0;
+    fa.one_window = nlc > 1 && (lc->sorted & (RR_SORTED_RUNS | RR_SORTED_TIMES | RR_ONE_WINDOW)) ==
+                                   (RR_SORTED_RUNS | RR_SORTED_TIMES | RR_ONE_WINDOW);
     lds = flux_lds_bytes(mixed, tab->ng, fa.chunk_pts, fa.uniform_ns, fa.row_lds != 0, nlc > 1);
     return RR_OK;
 }

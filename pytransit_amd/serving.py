@@ -36,7 +36,7 @@ import torch
 from . import _lib
 from ._lib import LD_LAW_IDS, RRLcData, RRLd, RRParams, RRRequest, check
 from .rrmodel import _PRECISIONS, RoadRunnerModel
-from .transitmodel import TransitModel, runs_sorted
+from .transitmodel import TransitModel, layout_flags
 
 __all__ = ['FreeList', 'Dataset', 'Request', 'RRServer']
 
@@ -117,7 +117,7 @@ class Dataset:
         self.npt, self.nlc, self.npb = tm.npt, tm.nlc, tm.npb
         self.nep = int(np.unique(tm.epids).size) if tm.epids.size else 1
         self.max_epid = int(tm.epids.max()) if tm.epids.size else 0
-        self.sorted = runs_sorted(tm.time, tm.lcids)
+        self.sorted = layout_flags(tm.time, tm.lcids, tm.epids, tm._per_lc(tm.exptimes, np.float64))
         self.uniform_ns = int(nsam[0]) if (nsam == nsam[0]).all() else 0
         self.host = OrderedDict(times=np.ascontiguousarray(tm.time, np.float64),
                                 lcids=tm.lcids.astype(np.int32), pbids=tm.pbids.astype(np.int32),

@@ -22,6 +22,21 @@ def runs_sorted(time: ndarray, lcids: ndarray) -> bool:
     return bool(np.all(dl >= 0) and np.all(np.diff(time)[dl == 0] >= 0))
 
 
+def layout_flags(time: ndarray, lcids: ndarray, epids: ndarray, exptimes: ndarray) -> int:
+    """The rr_lcdata.sorted bits: 1 (RR_SORTED_RUNS) when runs_sorted; with it 2
+    (RR_SORTED_TIMES) when the times are sorted over the whole array and 4
+    (RR_ONE_WINDOW) when every light curve has the same epoch index and exposure
+    time, i.e. a pv has one transit window for all light curves."""
+    if not runs_sorted(time, lcids):
+        return 0
+    flags = 1
+    if time.size < 2 or bool(np.all(np.diff(time) >= 0)):
+        flags |= 2
+    if (epids.size == 0 or (epids == epids[0]).all()) and (exptimes.size == 0 or (exptimes == exptimes[0]).all()):
+        flags |= 4
+    return flags
+
+
 class TransitModel:
     """Exoplanet transit light curve model (transitmodel.py:36-54)."""
 
@@ -122,7 +137,7 @@ class TransitModel:
             if (nsamples < 1).any():
                 raise ValueError('nsamples must be >= 1')
             d = dict(
-                sorted=runs_sorted(self.time, self.lcids),
+                sorted=layout_flags(self.time, self.lcids, self.epids, self._per_lc(self.exptimes, np.float64)),
                 uniform_ns=int(nsamples[0]) if (nsamples == nsamples[0]).all() else 0,
                 times=torch.as_tensor(np.ascontiguousarray(self.time), device=device),
                 lcids=torch.as_tensor(self.lcids.astype(np.int32), device=device),

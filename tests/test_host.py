@@ -147,7 +147,7 @@ def test_evaluate_descriptors_population(fake):
     assert out.shape == (npv, 50) and out.dtype == torch.float32
     (kind, prec, d), (kind2, _, _) = fake.calls
     assert kind == 'setup' and kind2 == 'flux' and prec == _lib.RR_MIXED
-    assert d == dict(npv=6, kcols=1, t0cols=1, npt=50, nlc=1, npb=1, nep=1, sorted=1, uns=10, law=9, nldc=2,
+    assert d == dict(npv=6, kcols=1, t0cols=1, npt=50, nlc=1, npb=1, nep=1, sorted=7, uns=10, law=9, nldc=2,
                      nk=256, ng=100, nz=40)
 
 
@@ -192,3 +192,25 @@ def test_product_path_fails_loudly_without_gpu():
     tm.set_data(np.linspace(-0.1, 0.1, 20))
     with pytest.raises(RuntimeError, match='GPU'):
         tm.evaluate(0.1, [0.3, 0.1], 0.0, 3.0, 8.0, 1.5)
+
+
+def test_layout_flags():
+    from pytransit_amd.transitmodel import layout_flags
+    t = np.linspace(0.0, 1.0, 12)
+    one = np.zeros(12, int)
+    two = np.repeat([0, 1], 6)
+    e1, x1 = np.zeros(1, int), np.zeros(1)
+    e2, x2 = np.zeros(2, int), np.zeros(2)
+    # one sorted light curve: all three bits
+    assert layout_flags(t, one, e1, x1) == 7
+    # unsorted times: nothing else counts
+    assert layout_flags(t[::-1].copy(), one, e1, x1) == 0
+    # two light curves in time order, one window
+    assert layout_flags(t, two, e2, x2) == 7
+    # the same times for each passband: runs sorted, times not
+    assert layout_flags(np.concatenate([t[:6], t[:6]]), two, e2, x2) == 1 | 4
+    # different exposure or epoch: no common window
+    assert layout_flags(t, two, e2, np.array([0.0, 0.01])) == 1 | 2
+    assert layout_flags(t, two, np.array([0, 1]), x2) == 1 | 2
+    # interleaved light curves: 0
+    assert layout_flags(t, np.tile([0, 1], 6), e2, x2) == 0
