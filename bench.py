@@ -19,8 +19,10 @@ Further legs, reported beside the headline in the same JSON line:
 * ``gradient``: fused log-likelihood with its adjoint gradient, 2,048 pv x 10,000
   times (mixed and FP64);
 * ``serving``: a Poisson stream of 1000 fit requests through the request server,
-  batched (4 slots, up to 16 requests per launch) and unbatched (one at a time),
-  with throughput and p50/p99 latency.
+  batched (4 slots, up to 16 requests and 2^28 points per launch) and unbatched
+  (one at a time), with throughput and p50/p99 latency; ``capacity`` offers the
+  same mix at twice the rate, which neither server keeps up with, and reports
+  what each serves.
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 3] [--no-cpu] [--no-serve] [--no-extra]
 """
@@ -160,6 +162,14 @@ def serving_leg(torch, dev, nreq, rate):
                      "p99_ms": float(np.percentile(r['latency'], 99) * 1e3),
                      "wall_s": r['wall'], "mean_batch": float(r['batch']), "slots": slots, "max_batch": mb,
                      "max_batch_points": mp}
+    # capacity: the same mix offered at twice the rate, which neither server keeps up with
+    st2 = workloads_mod().serving_stream(n_requests=nreq, rate=2 * rate)
+    cap = {"offered_requests_per_s": 2 * rate}
+    for name, slots, mb, mp in (('batched', 4, 16, 1 << 28), ('unbatched', 1, 1, None)):
+        r = serve_stream(torch, dev, st2, slots, mb, budget, mp)
+        cap[name + "_points_per_s"] = r['points'] / r['wall']
+        cap[name + "_requests_per_s"] = nreq / r['wall']
+    out["capacity"] = cap
     out["workload"] = (f"{nreq} fit requests, Poisson arrivals at {rate:g}/s; light-curve sets of log-uniform "
                        f"1k-200k points in 1-64 passbands (24-set pool, resident), log-uniform 32-8192 pv, "
                        f"quadratic/power-2/nonlinear-4 LD, nss in {{1,5,10}}; mixed precision; flux left on device")
