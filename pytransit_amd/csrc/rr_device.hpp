@@ -442,53 +442,40 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // 2*pi*trapz(z*I(mu), z) on mu = linspace(1, 0, 200) (rrmodel.py:151-152,220-227) by a
-// whole wavefront: each node's law value is computed once, segment m pairs node m
-// with node m-1 (from the next lower lane, or lane 63 of the previous round).
-__device__ __forceinline__ double ld_trapz_wave(int law, const double* c, int nc) {
+// whole wavefront, for the profile I = f(mu).  Each node's value z*I(mu) is computed
+// once into `s_y` (kTrapzNodes doubles of LDS), then segment m pairs node m with node
+// m-1, each lane summing its segments in node order before the wave sum.  Staging
+// the values in LDS keeps the law's evaluation (an FP64 pow for the power laws) out
+// of the summation loop, whose registers would otherwise spill.  Every lane of the
+// block must call it (it holds two barriers).
+constexpr int kTrapzNodes = 200;
+template <class F>
+__device__ __forceinline__ double trapz_disk_wave(F f, double* s_y) {
     const int lane = threadIdx.x & 63;
-    double part = 0.0, carry_z = 0.0, carry_y = 0.0;
-    for (int base = 0; base < 200; base += 64) {
-        const int m = base + lane;
-        double z = 0.0, y = 0.0;
-        if (m < 200) {
-            const double mu = linspace_at(1.0, 0.0, 200, m);
-            z = sqrt(1.0 - mu * mu);
-            y = z * ld_law(law, mu, c, nc);
-        }
-        double zl = __shfl_up(z, 1, 64), yl = __shfl_up(y, 1, 64);
-        if (lane == 0) {
-            zl = carry_z;
-            yl = carry_y;
-        }
-        if (m >= 1 && m < 200) part += (z - zl) * (y + yl) / 2.0;
-        carry_z = __shfl(z, 63, 64);
-        carry_y = __shfl(y, 63, 64);
+    for (int m = lane; m < kTrapzNodes; m += 64) {
+        const double mu = linspace_at(1.0, 0.0, kTrapzNodes, m);
+        s_y[m] = sqrt(1.0 - mu * mu) * f(mu);
     }
+    __syncthreads();
+    double part = 0.0;
+    for (int m = lane; m < kTrapzNodes; m += 64) {
+        if (m >= 1) {
+            const double mu = linspace_at(1.0, 0.0, kTrapzNodes, m);
+            const double ml = linspace_at(1.0, 0.0, kTrapzNodes, m - 1);
+            const double z = sqrt(1.0 - mu * mu), zl = sqrt(1.0 - ml * ml);
+            part += (z - zl) * (s_y[m] + s_y[m - 1]) / 2.0;
+        }
+    }
+    __syncthreads();
     return 2.0 * kPi * wave_sum(part);
 }
 
-// 2*pi*trapz(z*B_i(mu), z) on mu = linspace(1, 0, 200), as ld_trapz_wave for a law.
-__device__ __forceinline__ double ld_basis_trapz_wave(int law, int i) {
-    const int lane = threadIdx.x & 63;
-    double part = 0.0, carry_z = 0.0, carry_y = 0.0;
-    for (int base = 0; base < 200; base += 64) {
-        const int m = base + lane;
-        double z = 0.0, y = 0.0;
-        if (m < 200) {
-            const double mu = linspace_at(1.0, 0.0, 200, m);
-            z = sqrt(1.0 - mu * mu);
-            y = z * ld_basis(law, i, mu);
-        }
-        double zl = __shfl_up(z, 1, 64), yl = __shfl_up(y, 1, 64);
-        if (lane == 0) {
-            zl = carry_z;
-            yl = carry_y;
-        }
-        if (m >= 1 && m < 200) part += (z - zl) * (y + yl) / 2.0;
-        carry_z = __shfl(z, 63, 64);
-        carry_y = __shfl(y, 63, 64);
-    }
-    return 2.0 * kPi * wave_sum(part);
+// The disk integral of a law's profile, and of its basis function i.
+__device__ __forceinline__ double ld_trapz_wave(int law, const double* c, int nc, double* s_y) {
+    return trapz_disk_wave([&](double mu) { return ld_law(law, mu, c, nc); }, s_y);
+}
+__device__ __forceinline__ double ld_basis_trapz_wave(int law, int i, double* s_y) {
+    return trapz_disk_wave([&](double mu) { return ld_basis(law, i, mu); }, s_y);
 }
 
 }  // namespace rr

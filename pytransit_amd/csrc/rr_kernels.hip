@@ -151,6 +151,7 @@ struct SetupArgs {
     PvRec* rec; double* istar; double* ldm64; float* ldm32;
     const double* t0; const int32_t* pbids; const int32_t* epids;   // light curve 0's context
     int npv, npb, kcols, t0cols, nk, ng, nz, n_orbit_blocks;
+    int ld_lds;                       // doubles of the LD role's profile area (<= kLdLds)
     double kmin, kmax, dk;
 };
 
@@ -409,9 +410,10 @@ __device__ __forceinline__ void ld_basis_role(const SetupArgs& s, int pv, int pb
     // disk integrals: analytic per passband, or the basis trapezoids (wave-wide)
     const bool analytic = law == RR_LD_UNIFORM || law == RR_LD_LINEAR || law == RR_LD_QUADRATIC ||
                           law == RR_LD_QUADRATIC_TRI;
+    double* s_y = sm + s.ld_lds + kLdGroup;  // trapezoid nodes
     double tb[kMaxBasis];
 #pragma unroll
-    for (int b = 0; b < kMaxBasis; ++b) tb[b] = (!analytic && b < nb) ? ld_basis_trapz_wave(law, b) : 0.0;
+    for (int b = 0; b < kMaxBasis; ++b) tb[b] = (!analytic && b < nb) ? ld_basis_trapz_wave(law, b, s_y) : 0.0;
     __syncthreads();
     for (int q = lane; q < gn; q += 64) {
         double ist;
@@ -500,32 +502,44 @@ __device__ __forceinline__ void ld_basis_role(const SetupArgs& s, int pv, int pb
     }
 }
 
+// kMulti = false serves launches whose requests all have one passband (every group
+// holds one); it leaves out the group and basis forms, whose registers would make
+// the one-passband blocks spill to scratch.
+template <bool kMulti>
 __device__ __forceinline__ void setup_ld_role(const SetupArgs& s, int pv, int grp, double* sm) {
-    double* s_l = sm;                // kLdLds
-    double* s_ist = sm + kLdLds;     // kLdGroup
+    double* s_l = sm;                // s.ld_lds
+    double* s_ist = sm + s.ld_lds;   // kLdGroup
     const int lane = threadIdx.x;
     const int nz = s.nz, ng = s.ng;
-    const int gs = ld_group_size(nz, s.law, s.nldc, s.npb);
-    const int pb0 = grp * gs;
-    const int gn = min(gs, s.npb - pb0);
+    // one passband: a single group of one, with no group arithmetic on the path
+    const int gs = kMulti ? ld_group_size(nz, s.law, s.nldc, s.npb) : 1;
+    const int pb0 = kMulti ? grp * gs : 0;
+    const int gn = kMulti ? min(gs, s.npb - pb0) : 1;
+    // k0 picks the table rows; its load is issued here and first waited on after
+    // the profiles and their integrals, which do not need it, so a cold start's
+    // memory latency overlaps that work.
     const double k0 = s.k[(size_t)pv * s.kcols];
-    const bool in_table = s.W != nullptr && s.kmin <= k0 && k0 <= s.kmax;
     int ik = 0;
     double ak = 0.0;
-    if (in_table) {
-        ik = (int)floor((k0 - s.kmin) / s.dk);
-        ik = ik > s.nk - 2 ? s.nk - 2 : ik;
-        ak = (k0 - s.kmin - ik * s.dk) / s.dk;
-    }
+    auto table_pos = [&]() {
+        const bool in = s.W != nullptr && s.kmin <= k0 && k0 <= s.kmax;
+        if (in) {
+            ik = (int)floor((k0 - s.kmin) / s.dk);
+            ik = ik > s.nk - 2 ? s.nk - 2 : ik;
+            ak = (k0 - s.kmin - ik * s.dk) / s.dk;
+        }
+        return in;
+    };
     const size_t j0 = (size_t)pv * s.npb + pb0;
-    if (ld_basis_group(nz, s.law, s.nldc, s.npb) > 0) {
+    if (kMulti && ld_basis_group(nz, s.law, s.nldc, s.npb) > 0) {
+        const bool in_table = table_pos();
         ld_basis_role(s, pv, pb0, gn, gs, j0, in_table, ik, ak, k0, sm);
         return;
     }
     // profiles at the mu nodes, lanes over (passband, node), and their disk integrals
     const bool tab = s.law == RR_LD_TABULATED;
     for (int i = lane; i < gn * nz; i += 64) {
-        const int q = i / nz, iz = i - q * nz;
+        const int q = kMulti ? i / nz : 0, iz = i - q * nz;
         const size_t j = j0 + q;
         s_l[i] = tab ? s.ldp_in[j * nz + iz] : ld_law(s.law, s.mu[iz], s.ldc + j * s.nldc, s.nldc);
     }
@@ -536,7 +550,7 @@ __device__ __forceinline__ void setup_ld_role(const SetupArgs& s, int pv, int gr
             ist = s.istar_in[j];
         } else {
             const double* c = s.ldc + j * s.nldc;
-            if (!ld_integral(s.law, c, ist)) ist = ld_trapz_wave(s.law, c, s.nldc);
+            if (!ld_integral(s.law, c, ist)) ist = ld_trapz_wave(s.law, c, s.nldc, s_ist + kLdGroup);
         }
         if (lane == 0) s_ist[q] = ist;
     }
@@ -549,6 +563,7 @@ __device__ __forceinline__ void setup_ld_role(const SetupArgs& s, int pv, int gr
     }
     // limb-darkening means: lerp in k of two table rows, or exact 2-D weights
     // when k0 is outside [kmin, kmax]; model_full.py:47-55
+    const bool in_table = table_pos();
     const bool mixed = s.ldm32 != nullptr;
     const void* wt = mixed ? (const void*)s.W32 : (const void*)s.W;
     const int kv = mixed ? 4 : 2;
@@ -558,29 +573,58 @@ __device__ __forceinline__ void setup_ld_role(const SetupArgs& s, int pv, int gr
         else ld_means_vec<double>(s, j0, ik, ak, s_l, s_l + nz);
     } else if (gn == 1 && in_table) ld_means_single(s, j0, ik, ak, s_l);
     else if (gn == 1) ld_means<1>(s, pv, j0, 1, in_table, ik, ak, k0, s_l);
-    else ld_means<kLdGroup>(s, pv, j0, gn, in_table, ik, ak, k0, s_l);
+    else if (kMulti) ld_means<kLdGroup>(s, pv, j0, gn, in_table, ik, ak, k0, s_l);
 }
 
-// Both roles share one LDS buffer, so a setup block holds only the larger of the two.
-constexpr int kSetupLds = kLdLds + kLdGroup > kOrbitLds ? kLdLds + kLdGroup : kOrbitLds;
+// Doubles of the LD role's profile area for one launch: the group's profiles
+// (gs x nz), the 16-byte path's two row sums (nz + 2 ng, taken by any group of one
+// passband), or the basis form's profiles, coefficients and integrals.
+__host__ __device__ inline int ld_lds_doubles(int nz, int ng, int law, int nldc, int npb) {
+    const int gs = ld_group_size(nz, law, nldc, npb);
+    int n = gs * nz;
+    if (nz + 2 * ng <= kLdLds && nz + 2 * ng > n) n = nz + 2 * ng;
+    if (ld_basis_group(nz, law, nldc, npb) > 0) {
+        const int nb = ld_basis_count(law, nldc);
+        const int m = nb * nz + gs * nb + gs;
+        n = m > n ? m : n;
+    }
+    return n < kLdLds ? n : kLdLds;
+}
 
+// Both roles share one dynamic LDS buffer of setup_lds_bytes: a block holds only
+// the larger of the two, sized to the launch (3.6 KB for one passband and the
+// default grids), so LDS does not cap the number of resident setup blocks.  The
+// LD role's layout: profile area [ld_lds], disk integrals [kLdGroup], trapezoid
+// nodes [kTrapzNodes].
+static size_t setup_lds_bytes(const SetupArgs& s) {
+    const int ld = s.ld_lds + kLdGroup + kTrapzNodes;
+    const int n = ld > kOrbitLds ? ld : kOrbitLds;
+    return sizeof(double) * (size_t)n;
+}
+
+template <bool kMulti>
 __device__ __forceinline__ void setup_body(const SetupArgs& s, int blk) {
-    __shared__ double sm[kSetupLds];
+    extern __shared__ double sm[];
     if (blk < s.n_orbit_blocks) {
         setup_orbit_role(s, blk, sm);
     } else {
         const int b = blk - s.n_orbit_blocks;
-        const int gsz = ld_group_size(s.nz, s.law, s.nldc, s.npb);
-        const int ngrp = (s.npb + gsz - 1) / gsz;
-        const int pv = b / ngrp;
-        setup_ld_role(s, pv, b - pv * ngrp, sm);
+        if (kMulti) {
+            const int gsz = ld_group_size(s.nz, s.law, s.nldc, s.npb);
+            const int ngrp = (s.npb + gsz - 1) / gsz;
+            const int pv = b / ngrp;
+            setup_ld_role<kMulti>(s, pv, b - pv * ngrp, sm);
+        } else {
+            setup_ld_role<kMulti>(s, b, 0, sm);  // one group per pv
+        }
     }
 }
 
-// Five waves per SIMD: with the block's 8.3 KB of LDS, 19 blocks fit a CU, so the
+// Five waves per SIMD (96 VGPRs): 20 blocks fit a CU when LDS does not bind, so the
 // 4096 + 512 blocks of a 4096-pv setup are resident at once on 256 CUs.
+template <bool kMulti>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) rr_setup_kernel(SetupArgs s) {
-    setup_body(s, blockIdx.x);
+    setup_body<kMulti>(s, blockIdx.x);
 }
 
 // Several requests in one launch: item i owns blocks [start[i], start[i+1]).  The
@@ -599,9 +643,10 @@ __device__ __forceinline__ int batch_item(const int* start, int n, int bid) {
     return i;
 }
 
+template <bool kMulti>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) rr_setup_batch_kernel(SetupBatch b) {
     const int i = batch_item(b.start, b.n, blockIdx.x);
-    setup_body(b.a[i], blockIdx.x - b.start[i]);
+    setup_body<kMulti>(b.a[i], blockIdx.x - b.start[i]);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1552,6 +1597,7 @@ static int prepare_setup(const char* who, int precision, const rr_table* tab, co
     sa.npv = npv; sa.npb = npb; sa.kcols = pv->kcols; sa.nk = tab->nk; sa.ng = tab->ng; sa.nz = tab->nz;
     sa.kmin = tab->kmin; sa.kmax = tab->kmax; sa.dk = tab->dk;
     sa.n_orbit_blocks = (npv + kOrbitGroup - 1) / kOrbitGroup;
+    sa.ld_lds = ld_lds_doubles(tab->nz, tab->ng, ld->law, ld->nldc, npb);
     const int gs = ld_group_size(tab->nz, ld->law, ld->nldc, npb);
     const long long nb = (long long)sa.n_orbit_blocks + (long long)npv * ((npb + gs - 1) / gs);
     if (nb > 0x7fffffffLL) return set_error(RR_ERANGE, "%s: setup grid too large", who);
@@ -1600,7 +1646,10 @@ int rr_setup(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_p
     int nblocks = 0;
     int rc = prepare_setup("rr_setup", precision, tab, lc, pv, ld, workspace, workspace_bytes, sa, nblocks);
     if (rc || nblocks == 0) return rc;
-    hipLaunchKernelGGL(rr_setup_kernel, dim3(nblocks), dim3(64), 0, (hipStream_t)stream, sa);
+    if (sa.npb > 1)
+        hipLaunchKernelGGL(rr_setup_kernel<true>, dim3(nblocks), dim3(64), setup_lds_bytes(sa), (hipStream_t)stream, sa);
+    else
+        hipLaunchKernelGGL(rr_setup_kernel<false>, dim3(nblocks), dim3(64), setup_lds_bytes(sa), (hipStream_t)stream, sa);
     return check_launch("rr_setup_kernel");
 }
 
@@ -1639,7 +1688,8 @@ int rr_evaluate_batch(int precision, const rr_table* tab, int nreq, const rr_req
     if (nreq > 0 && !reqs) return set_error(RR_EINVAL, "rr_evaluate_batch: null request array");
     SetupBatch sb;
     FluxBatch fb[2];  // [0] general, [1] contiguous sorted light curves
-    size_t lds[2] = {0, 0};
+    size_t lds[2] = {0, 0}, slds = 0;
+    bool multi = false;
     long long fblocks[2] = {0, 0};
     long long sblocks = 0;
     sb.n = 0;
@@ -1663,6 +1713,8 @@ int rr_evaluate_batch(int precision, const rr_table* tab, int nreq, const rr_req
         sb.start[sb.n] = (int)sblocks;
         sb.a[sb.n++] = sa;
         sblocks += nb;
+        slds = setup_lds_bytes(sa) > slds ? setup_lds_bytes(sa) : slds;
+        multi = multi || sa.npb > 1;
         FluxBatch& b = fb[fast ? 1 : 0];
         b.start[b.n] = (int)fblocks[fast ? 1 : 0];
         b.a[b.n++] = fa;
@@ -1674,7 +1726,8 @@ int rr_evaluate_batch(int precision, const rr_table* tab, int nreq, const rr_req
     if (sb.n == 0) return RR_OK;
     hipStream_t s = (hipStream_t)stream;
     sb.start[sb.n] = (int)sblocks;
-    hipLaunchKernelGGL(rr_setup_batch_kernel, dim3((unsigned)sblocks), dim3(64), 0, s, sb);
+    if (multi) hipLaunchKernelGGL(rr_setup_batch_kernel<true>, dim3((unsigned)sblocks), dim3(64), slds, s, sb);
+    else hipLaunchKernelGGL(rr_setup_batch_kernel<false>, dim3((unsigned)sblocks), dim3(64), slds, s, sb);
     int rc = check_launch("rr_setup_batch_kernel");
     if (rc) return rc;
     const bool mixed = precision == RR_MIXED;
