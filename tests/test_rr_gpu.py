@@ -258,6 +258,34 @@ def test_linear_laws_many_passbands_basis(gpu, oracle, law, npb, nldc):
         assert err <= tol
 
 
+@pytest.mark.parametrize('nzin,nzlimb,npb', [(64, 64, 9), (400, 400, 1), (400, 400, 3)])
+def test_setup_lds_layout_large_z_grids(gpu, oracle, nzin, nzlimb, npb):
+    # The setup block's LDS is sized per launch.  nz = 128 with 9 passbands fills
+    # the profile area (8 x 128 doubles) and leaves a last group of one passband
+    # on the 16-byte path; nz = 800 leaves room for one profile plus that path's
+    # two row sums (nz + 2 ng = 1000 of 1024 doubles), with the disk integrals
+    # and the 200 trapezoid nodes after them.
+    say(f'case: power-2, nz = {nzin + nzlimb}, {npb} passband(s), nss=3, setup LDS layout')
+    npv = 12
+    wl = workloads.rr_bjd(npv=npv, npt=2000, seed=13)
+    wl.nsamples = 3
+    lcids = pbids = None
+    if npb > 1:
+        rng = np.random.default_rng(13)
+        lcids = (np.arange(wl.times.size) * npb) // wl.times.size
+        pbids = np.arange(npb)
+        wl.k = wl.k + 0.002 * np.sin(np.arange(npb) / 3.0)[None, :]
+        wl.ldc = wl.ldc[:, :1, :] + rng.uniform(-0.03, 0.03, (npv, npb, 2))
+    fo = oracle_flux(oracle, wl, 'power-2', nzin=nzin, nzlimb=nzlimb, lcids=lcids, pbids=pbids)
+    assert (fo < 1).sum() > 300
+    for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
+        fg = gpu_flux(wl, prec, lcids=lcids, pbids=pbids, nzin=nzin, nzlimb=nzlimb)
+        assert np.array_equal(np.isnan(fo), np.isnan(fg)), prec
+        err = np.nanmax(np.abs(fg - fo))
+        say(f'  {prec}: max |dF| = {err:.3e}')
+        assert err <= tol
+
+
 def test_short_periods_window_search(gpu, oracle):
     # periods of 0.45-1.6 d against 1.42-d tiles (1024 times at 2-min cadence):
     # tiles spanning one, two (window ranges found by search) and three or more
