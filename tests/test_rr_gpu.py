@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 import torch
 
-from pytransit_amd import RoadRunnerModel, _lib, workloads
+from pytransit_amd import RoadRunnerModel, TabulatedLDModel, _lib, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -315,8 +315,36 @@ def test_k_outside_table_uses_exact_weights(gpu, oracle):
     wl.k = wl.k.copy()
     wl.k[:4, 0] = [0.003, 0.0045, 0.55, 0.6]
     fo = oracle_flux(oracle, wl, 'quadratic')
-    fg = gpu_flux(wl, 'fp64')
-    assert np.nanmax(np.abs(fg - fo)) <= FP64_TOL
+    for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
+        fg = gpu_flux(wl, prec)
+        err = np.nanmax(np.abs(fg - fo))
+        say(f'  {prec}: max |dF| = {err:.3e}')
+        assert err <= tol
+
+
+def test_tabulated_profiles_reproduce_the_law(gpu, oracle):
+    # Profiles given as a table at the model's mu nodes (an LDModel), here the
+    # quadratic law's own values with its analytic disk integral, give the
+    # quadratic model: the tabulated-profile branch of the setup against the oracle.
+    say('case: tabulated LD profiles (quadratic values at the mu nodes), one passband')
+    wl = workloads.rr_small(npv=8)
+    wl.k = wl.k.copy()
+    wl.k[0, 0] = 0.55  # off the table: exact 2-D weights on tabulated profiles
+    fo = oracle_flux(oracle, wl, 'quadratic')
+    c = np.asarray(wl.ldc, np.float64).reshape(wl.k.shape[0], 1, 2)
+    for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
+        probe = RoadRunnerModel('quadratic', precision=prec)
+        mu = np.asarray(probe.mu, np.float64)
+        ldp = np.stack([oracle.ld_quadratic(mu, c[j, 0]) for j in range(c.shape[0])])[:, None, :]
+        istar = np.array([[oracle.ldi_quadratic(c[j, 0])] for j in range(c.shape[0])])
+        tm = RoadRunnerModel(TabulatedLDModel(ldp, istar), precision=prec)
+        tm.set_data(wl.times, nsamples=wl.nsamples, exptimes=wl.exptimes)
+        fg = np.asarray(tm.evaluate(wl.k, wl.ldc, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w), np.float64)
+        fg = fg.reshape(-1, len(wl.times))
+        assert np.array_equal(np.isnan(fo), np.isnan(fg)), prec
+        err = np.nanmax(np.abs(fg - fo))
+        say(f'  {prec}: max |dF| = {err:.3e}')
+        assert err <= tol
 
 
 def test_single_pv_scalars(gpu, oracle):
