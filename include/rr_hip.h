@@ -149,6 +149,19 @@ int rr_ld_profiles(int law, const double* ldc, int npv, int npb, int nldc,
 int rr_integrate_profiles(const double* mu, int nmu, const double* ldp, int npv, int npb,
                           double* istar, void* stream);
 
+/* Profiles ldp[npv, npb, nmu] interpolated trilinearly in a grid of stellar
+ * intensity profiles grid[nx, ny, nz, npb, nmu] (teff, logg, metallicity axes,
+ * evenly spaced: axis node i at x0 + i dx) at each pv's parameters
+ * (par[pv * par_stride + 0..2] = teff, logg, metallicity).  Parameters are
+ * clamped to the grid; a NaN parameter gives a NaN profile row.  Each axis needs
+ * at least 2 nodes.  Replaces trilinear_interpolation_set
+ * (models/numba/ldtkldm.py:5-57) as called by LDTkLDModel.__call__
+ * (models/ldtkldm.py:74-89); the disk integrals then come from
+ * rr_integrate_profiles. */
+int rr_ldtk_interpolate(const double* grid, int nx, int ny, int nz, int npb, int nmu, double x0, double dx,
+                        double y0, double dy, double z0, double dz, const double* par, int par_stride, int npv,
+                        double* ldp, void* stream);
+
 /* Bytes of device workspace rr_evaluate / rr_setup / rr_flux need for these sizes:
  * the per-pv state (orbit coefficients, contact window, validity), the stellar
  * disk integrals and the limb-darkening means. */

@@ -326,6 +326,40 @@ def integrate_profiles(mu: np.ndarray, ldp: np.ndarray) -> np.ndarray:
     return 2.0 * PI * np.cumsum(s, axis=-1)[..., -1]
 
 
+def _grid_cell(x: float, x0: float, dx: float, n: int):
+    """Cell index and fraction along one evenly spaced axis; numba/ldtkldm.py:40-51.
+    The value is clamped to the axis and the index to n - 2, so the last node is
+    reached with a fraction of 1.  (The reference clamps to x0 + n dx and then
+    slices cell n - 1, which reads past its grid at and above the last node; inside
+    the grid the two are the same.)"""
+    x = min(max(x, x0), x0 + (n - 1) * dx)
+    i = int(math.floor((x - x0) / dx))
+    i = min(max(i, 0), n - 2)
+    return i, (x - x0 - i * dx) / dx
+
+
+def trilinear_interpolation_set(data, xs, ys, zs, x0, dx, y0, dy, z0, dz):
+    """Profiles [npv, npb, nmu] interpolated trilinearly in data[nx, ny, nz, npb, nmu]
+    at each (xs, ys, zs); numba/ldtkldm.py:5-57, with the corner weights and the
+    sum in the order of ldtkldm.py:14-35.  A NaN parameter gives a NaN row."""
+    nx, ny, nz = data.shape[:3]
+    out = np.empty((len(xs),) + data.shape[3:])
+    for j, (x, y, z) in enumerate(zip(xs, ys, zs)):
+        if math.isnan(x) or math.isnan(y) or math.isnan(z):
+            out[j] = np.nan
+            continue
+        ix, ax = _grid_cell(x, x0, dx, nx)
+        iy, ay = _grid_cell(y, y0, dy, ny)
+        iz, az = _grid_cell(z, z0, dz, nz)
+        d = data[ix:ix + 2, iy:iy + 2, iz:iz + 2]
+        rx, ry, rz = 1.0 - ax, 1.0 - ay, 1.0 - az
+        a1, a2, b1, b2 = rx * ry * rz, ax * ry * rz, rx * ay * rz, rx * ry * az
+        c1, c2, d1, d2 = ax * ry * az, rx * ay * az, ax * ay * rz, ax * ay * az
+        out[j] = (d[0, 0, 0] * a1 + d[1, 0, 0] * a2 + d[0, 1, 0] * b1 + d[0, 0, 1] * b2 +
+                  d[1, 0, 1] * c1 + d[0, 1, 1] * c2 + d[1, 1, 0] * d1 + d[1, 1, 1] * d2)
+    return out
+
+
 # --------------------------------------------------------------------------------------
 # Orbit: Kepler solve and Taylor-series sky-plane position (orbits_py.py, taylor_z.py)
 # --------------------------------------------------------------------------------------

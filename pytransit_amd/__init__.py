@@ -5,7 +5,7 @@ Public API mirrors ``pytransit`` (``pytransit/__init__.py:64-66``):
 ``TSModel``), ``QuadraticModel`` (Mandel & Agol) and the ``LDModel`` plug-in
 protocol; ``RoadRunnerLPFModel`` takes the layouts of PyTransit's LPFs.
 """
-from .ldmodel import LDModel, TabulatedLDModel
+from .ldmodel import LDModel, LDTkLDModel, TabulatedLDModel
 from .lpf import RoadRunnerLPFModel
 from .qmodel import QuadraticModel
 from .rrmodel import RoadRunnerModel, RRModel
@@ -14,4 +14,4 @@ from .tsmodel import TransmissionSpectroscopyModel, TSModel
 __version__ = '0.1.0'
 
 __all__ = ['RoadRunnerModel', 'RRModel', 'TransmissionSpectroscopyModel', 'TSModel', 'QuadraticModel',
-           'RoadRunnerLPFModel', 'LDModel', 'TabulatedLDModel']
+           'RoadRunnerLPFModel', 'LDModel', 'TabulatedLDModel', 'LDTkLDModel']

@@ -68,6 +68,8 @@ _SIGNATURES = {
                               POINTER(c_double), POINTER(c_double), c_void_p]),
     'rr_ld_profiles': (c_int, [c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     'rr_integrate_profiles': (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    'rr_ldtk_interpolate': (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_double, c_double, c_double,
+                                    c_double, c_double, c_double, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     'rr_orbit': (c_int, [c_int] + [c_void_p] * 9),
     'rr_workspace_bytes': (c_size_t, [c_int, c_int, c_int, c_int, c_int]),
     'rr_setup': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd), c_void_p,

@@ -127,6 +127,55 @@ __global__ void __launch_bounds__(256) integrate_profiles_kernel(const double* _
     istar[j] = 2.0 * kPi * s;
 }
 
+// ------------------------------------------------------------------------------------
+// Trilinear interpolation of a profile grid over (teff, logg, metallicity) for the
+// rr_ldtk_interpolate entry point; ldtkldm.py:5-57.  One lane per output value
+// ldp[pv][pb][mu]: the lane finds its pv's cell and fractions (a few operations)
+// and reads the cell's eight corners, which the lanes of one pv share through L2.
+// ------------------------------------------------------------------------------------
+
+// Cell index and fraction along one evenly spaced axis of n >= 2 nodes
+// (ldtkldm.py:40-51).  The value is clamped to the axis, and the cell index to
+// n - 2, so the last node is reached with a fraction of 1.  (The reference clamps
+// the value to x0 + n dx and then indexes cell n - 1, which reads past its grid at
+// and above the last node; inside the grid the two are the same.)
+__device__ __forceinline__ void grid_cell(double x, double x0, double dx, int n, int& i, double& a) {
+    x = fmin(fmax(x, x0), x0 + (n - 1) * dx);
+    i = (int)floor((x - x0) / dx);
+    i = min(max(i, 0), n - 2);
+    a = (x - x0 - i * dx) / dx;
+}
+
+__global__ void __launch_bounds__(256) ldtk_interpolate_kernel(const double* __restrict__ grid, int nx, int ny,
+                                                               int nz, int nq, double x0, double dx, double y0,
+                                                               double dy, double z0, double dz,
+                                                               const double* __restrict__ par, int par_stride,
+                                                               int npv, double* __restrict__ ldp) {
+#pragma clang fp contract(off)
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)npv * nq) return;
+    const int pv = (int)(idx / nq), q = (int)(idx - (long long)pv * nq);
+    const double x = par[(size_t)pv * par_stride], y = par[(size_t)pv * par_stride + 1],
+                 z = par[(size_t)pv * par_stride + 2];
+    if (isnan(x) || isnan(y) || isnan(z)) {
+        ldp[idx] = NAN;
+        return;
+    }
+    int ix, iy, iz;
+    double ax, ay, az;
+    grid_cell(x, x0, dx, nx, ix, ax);
+    grid_cell(y, y0, dy, ny, iy, ay);
+    grid_cell(z, z0, dz, nz, iz, az);
+    // the corner weights and the sum in the order of ldtkldm.py:14-35
+    const double rx = 1.0 - ax, ry = 1.0 - ay, rz = 1.0 - az;
+    const double a1 = rx * ry * rz, a2 = ax * ry * rz, b1 = rx * ay * rz, b2 = rx * ry * az;
+    const double c1 = ax * ry * az, c2 = rx * ay * az, d1 = ax * ay * rz, d2 = ax * ay * az;
+    const size_t sz = (size_t)nq, sy = sz * nz, sx = sy * ny;
+    const double* g = grid + (size_t)ix * sx + (size_t)iy * sy + (size_t)iz * sz + q;
+    ldp[idx] = g[0] * a1 + g[sx] * a2 + g[sy] * b1 + g[sz] * b2 + g[sx + sz] * c1 + g[sy + sz] * c2 +
+               g[sx + sy] * d1 + g[sx + sy + sz] * d2;
+}
+
 __global__ void __launch_bounds__(64) orbit_kernel(int npv, const double* __restrict__ P, const double* __restrict__ A,
                                                    const double* __restrict__ I, const double* __restrict__ E,
                                                    const double* __restrict__ Wp, const double* __restrict__ K0,
@@ -1482,6 +1531,25 @@ int rr_integrate_profiles(const double* mu, int nmu, const double* ldp, int npv,
     hipLaunchKernelGGL(integrate_profiles_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, mu, nmu,
                        ldp, n, istar);
     return check_launch("integrate_profiles_kernel");
+}
+
+int rr_ldtk_interpolate(const double* grid, int nx, int ny, int nz, int npb, int nmu, double x0, double dx,
+                        double y0, double dy, double z0, double dz, const double* par, int par_stride, int npv,
+                        double* ldp, void* stream) {
+    if (nx < 2 || ny < 2 || nz < 2 || npb < 1 || nmu < 1 || npv < 0 || par_stride < 3)
+        return set_error(RR_EINVAL, "rr_ldtk_interpolate: bad sizes (each axis needs >= 2 nodes, par_stride >= 3)");
+    if (!(dx > 0.0 && dy > 0.0 && dz > 0.0) || !std::isfinite(x0) || !std::isfinite(y0) || !std::isfinite(z0) ||
+        !std::isfinite(dx) || !std::isfinite(dy) || !std::isfinite(dz))
+        return set_error(RR_EINVAL, "rr_ldtk_interpolate: axis origins must be finite and steps positive");
+    if ((long long)npb * nmu > 0x7fffffffLL || (long long)npv * npb * nmu > 0x7fffffffLL * 256LL)
+        return set_error(RR_ERANGE, "rr_ldtk_interpolate: too many values");
+    if (npv == 0) return RR_OK;
+    if (!grid || !par || !ldp) return set_error(RR_EINVAL, "rr_ldtk_interpolate: null pointer");
+    const long long n = (long long)npv * npb * nmu;
+    hipLaunchKernelGGL(ldtk_interpolate_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, grid, nx, ny, nz, npb * nmu, x0, dx, y0, dy, z0, dz, par, par_stride,
+                       npv, ldp);
+    return check_launch("ldtk_interpolate_kernel");
 }
 
 int rr_orbit(int npv, const double* p, const double* a, const double* inc, const double* e, const double* w,

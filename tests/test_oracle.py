@@ -393,3 +393,68 @@ def test_mandel_agol_light_curve_matches_roadrunner(oracle):
     assert (fma < 1).sum() > 50 and (fma == 1.0).sum() > 50
     f2 = oracle.quadratic_model(t, [[np.nan]], [[t0]], [p], [a], [i], [0.0], [0.0], [u], one, [0], [0], [1], [0.0])
     assert np.isinf(f2).all()
+
+
+def _trilinear_grid(rng, nx=5, ny=4, nz=3, nq=(2, 7)):
+    """A grid whose values are trilinear in (x, y, z) with coefficients that vary
+    over the trailing (passband, mu) axes, and the function itself."""
+    x = 4000.0 + 250.0 * np.arange(nx)
+    y = 3.5 + 0.5 * np.arange(ny)
+    z = -0.5 + 0.25 * np.arange(nz)
+    co = rng.normal(size=(8,) + nq)
+
+    def f(a, b, c):
+        u, v, w = (a - 4000.0) / 1000.0, b - 4.0, c
+        return (co[0] + co[1] * u + co[2] * v + co[3] * w + co[4] * u * v + co[5] * u * w + co[6] * v * w +
+                co[7] * u * v * w)
+
+    data = np.empty((nx, ny, nz) + nq)
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                data[i, j, k] = f(x[i], y[j], z[k])
+    return x, y, z, data, f
+
+
+def test_trilinear_interpolation_reproduces_trilinear_functions(oracle):
+    """numba/ldtkldm.py:5-57: trilinear interpolation is exact for functions
+    trilinear in (teff, logg, metallicity) (a swapped corner weight or axis fails
+    this), returns the node values at the nodes, clamps outside the grid and gives
+    a NaN row for a NaN parameter."""
+    rng = np.random.default_rng(5)
+    x, y, z, data, f = _trilinear_grid(rng)
+    args = (x[0], x[1] - x[0], y[0], y[1] - y[0], z[0], z[1] - z[0])
+    xs = rng.uniform(x[0], x[-1], 50)
+    ys = rng.uniform(y[0], y[-1], 50)
+    zs = rng.uniform(z[0], z[-1], 50)
+    out = oracle.trilinear_interpolation_set(data, xs, ys, zs, *args)
+    want = np.stack([f(a, b, c) for a, b, c in zip(xs, ys, zs)])
+    assert np.abs(out - want).max() < 1e-11
+    # nodes, including the last node of every axis
+    nodes = [(0, 0, 0), (4, 3, 2), (2, 1, 1), (4, 0, 2)]
+    out = oracle.trilinear_interpolation_set(data, [x[i] for i, _, _ in nodes], [y[j] for _, j, _ in nodes],
+                                             [z[k] for _, _, k in nodes], *args)
+    for r, (i, j, k) in enumerate(nodes):
+        assert np.array_equal(out[r], data[i, j, k])
+    # clamped to the grid; a NaN parameter gives NaN
+    out = oracle.trilinear_interpolation_set(data, [3000.0, 9000.0, np.nan], [1.0, 9.0, 4.0], [-5.0, 5.0, 0.0],
+                                             *args)
+    assert np.array_equal(out[0], data[0, 0, 0])
+    assert np.array_equal(out[1], data[-1, -1, -1])
+    assert np.isnan(out[2]).all()
+
+
+def test_ldtk_model_needs_a_grid_without_ldtk():
+    """As the reference without LDTk (ldtkldm.py:11-15), building the model from
+    passbands and stellar parameter ranges alone raises ModuleNotFoundError; a grid
+    whose shape disagrees with its axes is rejected."""
+    from pytransit_amd import LDTkLDModel
+    with pytest.raises(ModuleNotFoundError):
+        LDTkLDModel(('a', 'b'), (5000, 100), (4.5, 0.1), (0.0, 0.1))
+    grid = np.ones((3, 2, 2, 1, 40))
+    m = LDTkLDModel(profiles=grid, teffs=[4000, 4500, 5000], loggs=[4.0, 4.5], metals=[-0.5, 0.0])
+    assert (m.nteff, m.nlogg, m.nmetal, m.npb, m.nmu) == (3, 2, 2, 1, 40)
+    with pytest.raises(ValueError):
+        LDTkLDModel(profiles=grid, teffs=[4000, 4500], loggs=[4.0, 4.5], metals=[-0.5, 0.0])
+    with pytest.raises(ValueError):
+        LDTkLDModel(profiles=grid, teffs=[4000, 4600, 5000], loggs=[4.0, 4.5], metals=[-0.5, 0.0])
