@@ -1,7 +1,7 @@
 """GPU parity of the LDTk-style profile path: trilinear interpolation of a grid of
 stellar intensity profiles over (teff, logg, metallicity) and its disk integrals
 (numba/ldtkldm.py:5-91, LDTkLDModel.__call__ of models/ldtkldm.py:74-89) against
-the oracle, and RoadRunnerModel driven by it.  LDTk itself is not installed: the
+the oracle, and RoadRunnerModel and TSModel driven by it.  LDTk itself is not installed: the
 grids here are synthetic, of the shape LDTk gives ([nteff, nlogg, nmetal, npb, nmu]
 at the model's mu nodes).  Bars: profiles and integrals to FP64 rounding; fluxes
 FP64 <= 1e-9, mixed <= 1 ppm."""
@@ -88,6 +88,35 @@ def test_roadrunner_with_ldtk_profiles(gpu, oracle):
             f = tm.evaluate(wl.k, par, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w)
             fs.append(np.asarray(f, np.float64).reshape(-1, wl.times.size))
         assert (fs[1] < 1).sum() > 500
+        err = np.abs(fs[0] - fs[1]).max()
+        say(f'  {prec}: max |dF| = {err:.3e}')
+        assert err <= tol
+
+
+def test_tsmodel_with_ldtk_profiles(gpu, oracle):
+    # the many-passband use LDTk grids are made for: TSModel with 16 passbands on
+    # 128 mu nodes, against the same profiles handed over as a table
+    from pytransit_amd import TSModel
+    say('case: TSModel(LDTkLDModel), 4 pv x 16 pb x 1500 t, 128 mu nodes')
+    rng = np.random.default_rng(11)
+    npv, npb = 4, 16
+    times = np.linspace(-0.3, 0.3, 1500)
+    mu = np.asarray(TSModel('quadratic', nzin=64, nzlimb=64).mu, np.float64)
+    grid = quadratic_grid(oracle, mu, npb)
+    par = stellar_params(rng, npv)
+    ldp_o = oracle.trilinear_interpolation_set(grid, par[:, 0], par[:, 1], par[:, 2], *AXES)
+    ist_o = oracle.integrate_profiles(mu, ldp_o)
+    k = 0.1 + 0.004 * np.sin(np.arange(npb))[None, :] + rng.uniform(-0.01, 0.01, (npv, 1))
+    t0, p, a, inc = rng.uniform(-0.02, 0.02, npv), np.full(npv, 3.0), np.full(npv, 10.0), np.full(npv, 1.56)
+    for prec, tol in (('fp64', 1e-12), ('mixed', MIXED_TOL)):
+        fs = []
+        for ldm in (LDTkLDModel(profiles=grid, teffs=TEFFS, loggs=LOGGS, metals=METALS),
+                    TabulatedLDModel(ldp_o, ist_o)):
+            tm = TSModel(ldm, nzin=64, nzlimb=64, precision=prec)
+            tm.set_data(times)
+            fs.append(np.asarray(tm.evaluate(k, par, t0, p, a, inc), np.float64))
+        assert fs[0].shape == (npv, npb, times.size)
+        assert (fs[1] < 1).sum() > 5000
         err = np.abs(fs[0] - fs[1]).max()
         say(f'  {prec}: max |dF| = {err:.3e}')
         assert err <= tol
