@@ -64,6 +64,19 @@ def test_c_abi_rejects_bad_arguments():
     reqs[1].pv.npv = -1
     rc = lib.rr_evaluate_batch(1, None, 2, reqs, None)
     assert rc == -1 and b'rr_evaluate_batch[0]' in lib.rr_last_error()
+    # profile-grid interpolation: axes of one node, a bad stride, a zero step and
+    # null pointers are rejected before any device work; an empty population is a no-op
+    rc = lib.rr_ldtk_interpolate(None, 1, 2, 2, 1, 4, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0, None, 3, 1, None, None)
+    assert rc == -1 and b'>= 2 nodes' in lib.rr_last_error()
+    rc = lib.rr_ldtk_interpolate(None, 2, 2, 2, 1, 4, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0, None, 2, 1, None, None)
+    assert rc == -1 and b'par_stride' in lib.rr_last_error()
+    rc = lib.rr_ldtk_interpolate(None, 2, 2, 2, 1, 4, 0.0, 0.0, 0.0, 1.0, 0.0, 1.0, None, 3, 1, None, None)
+    assert rc == -1 and b'steps positive' in lib.rr_last_error()
+    rc = lib.rr_ldtk_interpolate(None, 2, 2, 2, 1, 4, 0.0, 1.0, 0.0, 1.0, float('nan'), 1.0, None, 3, 1, None, None)
+    assert rc == -1 and b'finite' in lib.rr_last_error()
+    rc = lib.rr_ldtk_interpolate(None, 2, 2, 2, 1, 4, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0, None, 3, 1, None, None)
+    assert rc == -1 and b'null pointer' in lib.rr_last_error()
+    assert lib.rr_ldtk_interpolate(None, 2, 2, 2, 1, 4, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0, None, 3, 0, None, None) == 0
 
 
 # ---- set_data (transitmodel.py:56-125) ----------------------------------------------
