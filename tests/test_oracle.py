@@ -7,6 +7,9 @@ Reference KATs used (paths relative to the PyTransit checkout):
   tests/test_ma_quadratic_nb.py:23-62   eval_quad_z_s values (7 decimals)
   tests/test_limb_darkening.py:21-199   ldi_* vs quad, edge cases
   tests/test_z.py:23-68                 circular z at 7 times (4 decimals)
+
+The profile-grid interpolation (numba/ldtkldm.py:5-57) has no reference test; it
+is pinned by exactness on trilinear functions, node values, clamping and NaN rows.
 """
 import math
 
