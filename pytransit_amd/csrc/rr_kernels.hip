@@ -338,6 +338,62 @@ __device__ __forceinline__ void ld_means(const SetupArgs& s, int pv, size_t j0, 
     }
 }
 
+// Limb-darkening means of gn (<= G) passbands inside the table with 8-byte (FP32
+// table) or 16-byte (FP64 table) loads: a lane owns two neighbouring g, so one
+// round of 50 lanes covers ng = 100 and each table row batch is half the load
+// instructions of ld_means.  Per value the same lerp and the same sum over the mu
+// nodes in order as ld_means.  Needs an even ng and a table aligned to its load.
+template <typename T, int G>
+__device__ __forceinline__ void ld_means_pair(const SetupArgs& s, size_t j0, int gn, int ik, double ak,
+                                              const double* s_l) {
+    typedef T vec2 __attribute__((ext_vector_type(2)));
+    constexpr int kB = sizeof(T) == 4 ? 8 : 4;
+    const int lane = threadIdx.x;
+    const int nz = s.nz, ng = s.ng, nv = ng / 2;
+    const T* W = std::is_same<T, float>::value ? (const T*)s.W32 : (const T*)s.W;
+    const size_t o0 = (size_t)ik * nz * ng, o1 = o0 + (size_t)nz * ng;
+    const bool mixed = s.ldm32 != nullptr;
+    for (int v = lane; v < nv; v += 64) {
+        double d[G][2];
+#pragma unroll
+        for (int q = 0; q < G; ++q) d[q][0] = d[q][1] = 0.0;
+        for (int iz0 = 0; iz0 < nz; iz0 += kB) {
+            vec2 w0[kB], w1[kB];
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const size_t r = (size_t)min(iz0 + u, nz - 1) * ng + 2 * (size_t)v;
+                w0[u] = *(const vec2*)(W + o0 + r);
+                w1[u] = *(const vec2*)(W + o1 + r);
+            }
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                if (iz0 + u < nz) {
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const double wk = (1.0 - ak) * (double)w0[u][e] + ak * (double)w1[u][e];
+#pragma unroll
+                        for (int q = 0; q < G; ++q)
+                            if (q < gn) d[q][e] += wk * s_l[q * nz + iz0 + u];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            if (q < gn) {
+                const size_t o = (j0 + q) * ng + 2 * (size_t)v;
+                if (mixed) {
+                    s.ldm32[o] = (float)d[q][0];
+                    s.ldm32[o + 1] = (float)d[q][1];
+                } else {
+                    s.ldm64[o] = d[q][0];
+                    s.ldm64[o + 1] = d[q][1];
+                }
+            }
+        }
+    }
+}
+
 // One passband inside the table: each lane takes two g values per round, so twice
 // the table loads are in flight per batch (the per-block latency bounds setup).
 __device__ __forceinline__ void ld_means_single(const SetupArgs& s, size_t j, int ik, double ak, const double* s_l) {
@@ -622,7 +678,10 @@ __device__ __forceinline__ void setup_ld_role(const SetupArgs& s, int pv, int gr
         else ld_means_vec<double>(s, j0, ik, ak, s_l, s_l + nz);
     } else if (gn == 1 && in_table) ld_means_single(s, j0, ik, ak, s_l);
     else if (gn == 1) ld_means<1>(s, pv, j0, 1, in_table, ik, ak, k0, s_l);
-    else if (kMulti) ld_means<kLdGroup>(s, pv, j0, gn, in_table, ik, ak, k0, s_l);
+    else if (kMulti && in_table && wt != nullptr && ng % 2 == 0 && ((uintptr_t)wt & (mixed ? 7 : 15)) == 0) {
+        if (mixed) ld_means_pair<float, kLdGroup>(s, j0, gn, ik, ak, s_l);
+        else ld_means_pair<double, kLdGroup>(s, j0, gn, ik, ak, s_l);
+    } else if (kMulti) ld_means<kLdGroup>(s, pv, j0, gn, in_table, ik, ak, k0, s_l);
 }
 
 // Doubles of the LD role's profile area for one launch: the group's profiles
