@@ -491,6 +491,61 @@ __device__ __forceinline__ void ld_means_vec(const SetupArgs& s, size_t j, int i
     }
 }
 
+// The basis means of ld_basis_role inside the table with a lane owning two
+// neighbouring g (8-byte loads of the FP32 table, 16-byte of the FP64 one), then
+// every passband's means from them: the same lerp and sums per value as the
+// per-g loop, in half the dependent load rounds.  Needs an even ng and a table
+// aligned to its load.
+template <typename T>
+__device__ __forceinline__ void basis_means_pair(const SetupArgs& s, int nb, int gn, size_t j0, int ik, double ak,
+                                                 const double* s_B, const double* s_coef) {
+    typedef T vec2 __attribute__((ext_vector_type(2)));
+    constexpr int kB = sizeof(T) == 4 ? 8 : 4;
+    const int lane = threadIdx.x;
+    const int nz = s.nz, ng = s.ng, nv = ng / 2;
+    const T* W = std::is_same<T, float>::value ? (const T*)s.W32 : (const T*)s.W;
+    const size_t o0 = (size_t)ik * nz * ng, o1 = o0 + (size_t)nz * ng;
+    const bool mixed = s.ldm32 != nullptr;
+    for (int v = lane; v < nv; v += 64) {
+        double m[kMaxBasis][2];
+#pragma unroll
+        for (int b = 0; b < kMaxBasis; ++b) m[b][0] = m[b][1] = 0.0;
+        for (int iz0 = 0; iz0 < nz; iz0 += kB) {
+            vec2 w0[kB], w1[kB];
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                const size_t r = (size_t)min(iz0 + u, nz - 1) * ng + 2 * (size_t)v;
+                w0[u] = *(const vec2*)(W + o0 + r);
+                w1[u] = *(const vec2*)(W + o1 + r);
+            }
+#pragma unroll
+            for (int u = 0; u < kB; ++u) {
+                if (iz0 + u < nz) {
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const double wk = (1.0 - ak) * (double)w0[u][e] + ak * (double)w1[u][e];
+#pragma unroll
+                        for (int b = 0; b < kMaxBasis; ++b)
+                            if (b < nb) m[b][e] += wk * s_B[b * nz + iz0 + u];
+                    }
+                }
+            }
+        }
+        for (int q = 0; q < gn; ++q) {
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                double val = 0.0;
+#pragma unroll
+                for (int b = 0; b < kMaxBasis; ++b)
+                    if (b < nb) val += s_coef[q * nb + b] * m[b][e];
+                const size_t o = (j0 + q) * ng + 2 * (size_t)v + e;
+                if (mixed) s.ldm32[o] = (float)val;
+                else s.ldm64[o] = val;
+            }
+        }
+    }
+}
+
 // The limb-darkening role through the law's basis (ld_basis_count > 0): basis
 // profiles at the mu nodes, the basis means M_b(g) from the table rows (lerped in k
 // once per value) or the exact 2-D weights, then for each of the gn passbands
@@ -542,6 +597,12 @@ __device__ __forceinline__ void ld_basis_role(const SetupArgs& s, int pv, int pb
     }
     // basis means, lanes over g
     const bool mixed = s.ldm32 != nullptr;
+    const void* wt = mixed ? (const void*)s.W32 : (const void*)s.W;
+    if (in_table && wt != nullptr && ng % 2 == 0 && ((uintptr_t)wt & (mixed ? 7 : 15)) == 0) {
+        if (mixed) basis_means_pair<float>(s, nb, gn, j0, ik, ak, s_B, s_coef);
+        else basis_means_pair<double>(s, nb, gn, j0, ik, ak, s_B, s_coef);
+        return;
+    }
     for (int ig = lane; ig < ng; ig += 64) {
         double m[kMaxBasis];
 #pragma unroll
