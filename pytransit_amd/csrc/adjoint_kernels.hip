@@ -27,7 +27,7 @@ namespace rr {
 
 struct LnlArgs {
     const double* times; const double* obs; const double* sigma;
-    const int32_t* nsamples; const double* exptimes;
+    const int32_t* nsamples; const double* exptimes; const int32_t* epids;
     const double* k; const double* t0; const double* p; const double* a; const double* inc;
     const double* e; const double* w;
     int law, nldc; const double* ldc;
@@ -164,7 +164,9 @@ __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
     // ---- main loop over times ----------------------------------------------------
     const int ns = s.nsamples[0];
     const double ext = s.exptimes[0];
-    const double t0v = s.t0[(size_t)pv * s.t0cols];
+    // the light curve's epoch column, clamped as the evaluate path does
+    const int ep = min(max(s.epids[0], 0), s.t0cols - 1);
+    const double t0v = s.t0[(size_t)pv * s.t0cols + ep];
     const double lo = s_win[0] - (0.003 + ext), hi = s_win[1] + (0.003 + ext);
     const double inv_p = 1.0 / p;
     Real cx[5], cy[5], dcx[5], dcy[5];
@@ -294,14 +296,15 @@ int rr_lnlike(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_
     if (tab->ng < 2 || tab->ng > kLnlMaxNg || tab->nz < 1 || tab->nz > kLnlMaxNz || !tab->weights || !tab->mu)
         return set_error(RR_EINVAL, "rr_lnlike: bad weight table (ng <= %d, nz <= %d)", kLnlMaxNg, kLnlMaxNz);
     const int npv = pv->npv, npt = lc->npt;
-    if (npv < 0 || npt < 0) return set_error(RR_EINVAL, "rr_lnlike: bad sizes");
+    if (npv < 0 || npt < 0 || pv->t0cols < 1) return set_error(RR_EINVAL, "rr_lnlike: bad sizes");
     if (npv == 0) return RR_OK;
-    if (!lc->times || !lc->nsamples || !lc->exptimes || !obs || !sigma || !lnl)
+    if (!lc->times || !lc->nsamples || !lc->exptimes || !lc->epids || !obs || !sigma || !lnl)
         return set_error(RR_EINVAL, "rr_lnlike: null array");
     if (!pv->k || !pv->t0 || !pv->p || !pv->a || !pv->inc || !pv->e || !pv->w)
         return set_error(RR_EINVAL, "rr_lnlike: null parameter array");
     LnlArgs s;
     s.times = lc->times; s.obs = obs; s.sigma = sigma; s.nsamples = lc->nsamples; s.exptimes = lc->exptimes;
+    s.epids = lc->epids;
     s.k = pv->k; s.t0 = pv->t0; s.p = pv->p; s.a = pv->a; s.inc = pv->inc; s.e = pv->e; s.w = pv->w;
     s.law = ld->law; s.nldc = ld->nldc; s.ldc = ld->ldc;
     s.W = tab->weights; s.mu = tab->mu; s.lnl = lnl; s.grad = grad;

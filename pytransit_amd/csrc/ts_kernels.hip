@@ -81,9 +81,12 @@ __global__ void __launch_bounds__(64) ts_setup_kernel(TsArgs s) {
         r->p = s.p[q];
         r->inv_p = 1.0 / s.p[q];
         r->good = valid ? 1 : 0;
-        // the reference tests kmin <= kmean <= max(k) (its kmax is shadowed by max(k))
+        // The reference tests kmin <= kmean <= max(k) (its kmax is shadowed by max(k)),
+        // which passes for every kmean >= kmin and reads past the table above kmax.
+        // Defined here: the table serves kmean in [kmin, kmax] (the table's bounds);
+        // elsewhere the exact 2-D weights at kmean are used, as the RR setup does.
         const double km = sum / s.npb;
-        if (s.use_table && s.kmin <= km && km <= kmx) {
+        if (s.use_table && s.kmin <= km && km <= s.kmax_tab) {
             int ik = (int)floor((km - s.kmin) / s.dk);
             ik = ik > s.nk - 2 ? s.nk - 2 : ik;
             r->ik = ik;

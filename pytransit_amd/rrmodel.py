@@ -374,6 +374,8 @@ class RoadRunnerModel(TransitModel):
         npv, k, t0, (p, a, inc, e, w) = self._params(k, t0, p, a, i, e, w, device)
         if k.shape[1] != 1:
             raise ValueError('lnlike needs one radius ratio per parameter vector')
+        if self.epids.max() >= t0.shape[1]:
+            raise ValueError(f"t0 has {t0.shape[1]} columns but the epoch indices go up to {self.epids.max()}")
         ld, _keep = self._ld_descriptor(ldc, npv, 1, device)
         obs = _as_dev(obs, device).reshape(-1)
         if obs.numel() != self.npt:

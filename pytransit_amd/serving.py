@@ -273,7 +273,11 @@ class RRServer:
         """Step until every submitted request has finished."""
         t_end = None if timeout is None else time.perf_counter() + timeout
         while not self.idle:
-            self.step()
+            if self.step() == 0 and not self._batches and self._queue:
+                raise RuntimeError(
+                    f'request {self._queue[0].rid} cannot be admitted: nothing is in flight and the arena '
+                    f'({self._arena.in_use} of {self._arena.capacity} bytes in use) is held by results that '
+                    f'have not been released')
             if t_end is not None and time.perf_counter() > t_end:
                 raise TimeoutError('server did not drain in time')
 
@@ -319,31 +323,44 @@ class RRServer:
             self._free_slots.append(b.slot)
         self._batches = keep
 
-    def _ensure_resident(self, ds: Dataset) -> bool:
-        if ds.offset is not None:
-            return True
-        off = self._arena.alloc(ds.nbytes)
+    def _alloc(self, nbytes: int, keep: Dataset) -> Optional[int]:
+        """First-fit block of ``nbytes``; while none fits, evict the least recently
+        used idle dataset other than ``keep``.  Returns None when nothing is left to
+        evict.  A victim's upload may still be in flight on the upload stream, and
+        its block can be refilled at once by a slot stream, so the victim's upload
+        is waited for before its block is freed."""
+        off = self._arena.alloc(nbytes)
         while off is None:
-            idle = [d for d in self._datasets if d.offset is not None and d.users == 0 and d is not ds]
+            idle = [d for d in self._datasets if d.offset is not None and d.users == 0 and d is not keep]
             if not idle:
-                return False
+                return None
             victim = min(idle, key=lambda d: d.last_used)
+            self._exec.retire_dataset(victim)
             self._arena.free(victim.offset)
             victim.offset = None
             victim.ready = None
-            off = self._arena.alloc(ds.nbytes)
+            off = self._arena.alloc(nbytes)
+        return off
+
+    def _ensure_resident(self, ds: Dataset) -> bool:
+        if ds.offset is not None:
+            return True
+        off = self._alloc(ds.nbytes, ds)
+        if off is None:
+            return False
         ds.offset = off
         self._exec.upload_dataset(ds)
         return True
 
     def _place(self, r: Request) -> bool:
-        """Reserve the request's memory (dataset, params + workspace, flux)."""
+        """Reserve the request's memory (dataset, params + workspace, flux),
+        evicting idle datasets other than the request's own when needed."""
         if not self._ensure_resident(r.dataset):
             return False
-        off = self._arena.alloc(r.params.nbytes + r.ws_bytes)
+        off = self._alloc(r.params.nbytes + r.ws_bytes, r.dataset)
         if off is None:
             return False
-        foff = self._arena.alloc(r.flux_bytes)
+        foff = self._alloc(r.flux_bytes, r.dataset)
         if foff is None:
             self._arena.free(off)
             return False
@@ -432,6 +449,13 @@ class _HipExecutor:
         ds.ready = torch.cuda.Event()
         ds.ready.record(self.upload_stream)
         ds._staging = buf
+
+    @staticmethod
+    def retire_dataset(ds: Dataset) -> None:
+        """Before an evicted dataset's block is reused: its upload must be complete."""
+        if ds.ready is not None:
+            ds.ready.synchronize()
+        ds._staging = None
 
     def launch(self, slot: int, reqs: List[Request]):
         stream, staging = self.streams[slot], self.staging[slot]

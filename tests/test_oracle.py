@@ -352,6 +352,166 @@ def test_tsmodel_equal_k_reduces_to_rr(oracle):
         np.testing.assert_allclose(fts[0, ipb], frr[0], atol=1e-15)
 
 
+def _ts_uniform(oracle, k, times, p=3.0, a=8.0, inc=0.5 * math.pi):
+    """TSModel (exact weights at mean k) with uniform limb darkening: every LD-mean row
+    is a weight row sum (1) and I* = pi, so F = 1 - area / pi for each passband."""
+    ze, _ = oracle.create_z_grid(0.7, 20, 20)
+    npb = k.shape[1]
+    ldp = np.ones((1, npb, ze.size))
+    ist = np.full((1, npb), math.pi)
+    f = oracle.tsmodel_serial(times, k, [0.0], [p], [a], [inc], [0.0], [0.0], [1], [0.0], ldp, ist, None, None,
+                              0.005, 0.5, 100, None, ze)[0]
+    z = oracle.sep(times, oracle.taylor_coeffs(p, a, inc, 0.0, 0.0))
+    return f, z
+
+
+def test_tsmodel_per_passband_k_closed_forms(oracle):
+    """model_trspec.py:41-43,79-90 with k varying over passbands, pinned by closed forms.
+
+    With uniform LD the flux is 1 - area/pi.  Where the mean-k planet is fully inside
+    the star (A0 = pi kbar^2, kappa0 = pi):
+      z <= 1 - kmax          : area = A0 k^2/kbar^2   -> F = 1 - k^2 exactly;
+      1 - kmax < z <= 1 - kbar: area = A0 + (k - kbar) 2 kbar pi -> F = 1 - (2 k kbar - kbar^2).
+    On the limb the area is the first-order Taylor expansion of the lens area in k about
+    kbar: A(kbar) + (k - kbar) dA/dk, with dA/dk taken by central differences of the
+    acos-form lens area (pinned by special cases and brute force above).  An inverted or
+    unsquared area factor, a flipped extrapolation sign or a split at 1 - kbar fails.
+    """
+    k = np.array([[0.08, 0.095, 0.1, 0.105, 0.12]])
+    kb, km = k.mean(), k.max()
+    times = np.linspace(-0.12, 0.12, 4001)
+    f, z = _ts_uniform(oracle, k, times)
+    kc = k.T
+    r1 = z <= 1 - km
+    r2 = (z > 1 - km) & (z <= 1 - kb)
+    r3 = (z > 1 - kb + 1e-4) & (z < 1 + kb - 1e-4)
+    assert r1.sum() > 100 and r2.sum() > 20 and r3.sum() > 100
+    np.testing.assert_allclose(f[:, r1], np.broadcast_to(1 - kc ** 2, f[:, r1].shape), atol=1e-14, rtol=0)
+    np.testing.assert_allclose(f[:, r2], np.broadcast_to(1 - (2 * kc * kb - kb ** 2), f[:, r2].shape),
+                               atol=1e-14, rtol=0)
+    zz = z[r3]
+    area = lambda kk: np.array([oracle.cci_area(1.0, kk, x) for x in zz])
+    h = 1e-7
+    dadk = (area(kb + h) - area(kb - h)) / (2 * h)
+    expect = 1 - (area(kb)[None, :] + (kc - kb) * dadk[None, :]) / math.pi
+    np.testing.assert_allclose(f[:, r3], expect, atol=1e-9, rtol=0)
+    # the expansion approximates the exact per-passband lens area
+    exact = np.array([1 - area(kk) / math.pi for kk in k[0]])
+    assert np.abs(f[:, r3] - exact).max() < 5e-4
+    # outside the window (mean-k contacts + 0.0015 d) the flux is exactly 1
+    assert (f[:, z > 1 + kb + 0.05] == 1.0).all()
+
+
+def _ts_depth_fit(oracle, p, a, b, ks, iters=4):
+    """The reference's accuracy study (notebooks/tsmodel/test_accuracy.ipynb cells 2-3):
+    RR flux per passband (power-2 [0.4, 0.4], nzin = nzlimb = 50, ng = 100) is fitted
+    with TSModel over (a, b, k_1..k_n); Gauss-Newton stands in for its scipy minimize."""
+    times = np.linspace(-0.1, 0.1, 500)
+    ze, zm = oracle.create_z_grid(0.7, 50, 50)
+    ldc = np.tile([0.4, 0.4], (1, ks.size, 1))
+    ldp, ist = oracle.evaluate_ld('power-2', oracle.mu_nodes(zm), ldc)
+    inc = lambda a_, b_: math.acos(b_ / a_)
+
+    def ts(x):
+        return oracle.tsmodel_serial(times, x[None, 2:], [0.0], [p], [x[0]], [inc(x[0], x[1])], [0.0], [0.0], [1],
+                                     [0.0], ldp, ist, None, None, 0.005, 0.5, 100, None, ze)[0]
+    om = oracle.OracleRR('power-2', nzin=50, nzlimb=50)
+    om.set_data(times)
+    f0 = np.array([om.evaluate(np.array([[kk]]), ldc[:, j:j + 1], np.array([[0.0]]), p, a, inc(a, b))[0]
+                   for j, kk in enumerate(ks)])
+    x = np.r_[a, b, ks]
+    for _ in range(iters):
+        r = (ts(x) - f0).ravel()
+        jac = np.empty((r.size, x.size))
+        for j in range(x.size):
+            h = 1e-6 * max(1.0, abs(x[j])) if j < 2 else 1e-7
+            xp, xm = x.copy(), x.copy()
+            xp[j] += h
+            xm[j] -= h
+            jac[:, j] = ((ts(xp) - f0).ravel() - (ts(xm) - f0).ravel()) / (2 * h)
+        x = x + np.linalg.lstsq(jac, -r, rcond=None)[0]
+    return x, f0, ts(np.r_[a, b, ks])
+
+
+def test_tsmodel_matches_roadrunner_per_passband(oracle):
+    """TSModel against RoadRunner run passband by passband, on the reference's own
+    hot-Jupiter accuracy case (test_accuracy.ipynb cell 4, par_wasp_39: p = 4.1,
+    a = 11, b = 0.45, k = linspace(0.144, 0.154, 42)).
+
+    * Depths: refitting RR's flux with TSModel gives transit-depth errors
+      |k_fit^2 - k^2| of a few ppm, the '~1 ppm for Jupiter-sized planets' of
+      CHANGELOG.md:55-56 (the largest here is 2.9 ppm).
+    * The notebook's printed shifts of a and b (cell 9: -0.0018263, 0.00016431) are
+      reproduced to 1 %.
+    * The flux itself differs by < 100 ppm (the mean-k LD means); a flipped
+      extrapolation sign or an inverted area factor moves it by > 700 ppm.
+    """
+    ks = np.linspace(0.144, 0.154, 42)
+    x, f0, f1 = _ts_depth_fit(oracle, 4.1, 11.0, 0.45, ks)
+    assert np.abs(f1 - f0).max() < 1e-4
+    depth_err = x[2:] ** 2 - ks ** 2
+    assert np.abs(depth_err).max() < 3.5e-6
+    assert x[0] - 11.0 == pytest.approx(-0.0018263083886651543, rel=1e-2)
+    assert x[1] - 0.45 == pytest.approx(0.00016431292394630814, rel=1e-2)
+
+
+def test_rr_table_lerp_in_k(oracle):
+    """The RR table path (model_full.py:47-51): ik = floor((k0 - kmin)/dk), ak =
+    (k0 - kmin - ik dk)/dk with dk = (kmax - kmin)/nk (Q1), LD means (1-ak) W[ik] ldp +
+    ak W[ik+1] ldp.  Pinned by rows rebuilt with weights_2d at the rows' own k
+    (linspace(kmin, kmax, nk)): the flux is affine in the LD means, so at ak = 1/4 it is
+    exactly 3/4 of the flux with row ik alone plus 1/4 of the flux with row ik+1 alone,
+    and at ak ~ 0 it is the flux with row ik.  Swapping ak and 1-ak, or indexing with
+    (kmax-kmin)/(nk-1), fails."""
+    times = np.linspace(-0.12, 0.12, 301)
+    ldc = np.array([[[0.45, 0.2]]])
+    om = oracle.OracleRR('quadratic')
+    om.set_data(times)
+    kmin, kmax, nk = 0.005, 0.5, 256
+    dk = (kmax - kmin) / nk
+    rows = np.linspace(kmin, kmax, nk)
+    for ik, frac in ((49, 0.25), (130, 0.25), (49, 1e-9)):
+        k0 = kmin + (ik + frac) * dk
+        ak = (k0 - kmin - ik * dk) / dk
+        assert int(math.floor((k0 - kmin) / dk)) == ik and abs(ak - frac) < 1e-9
+
+        def flux_with(table):
+            om_t = oracle.OracleRR('quadratic')
+            om_t.set_data(times)
+            om_t.weights = table
+            return om_t.evaluate(np.array([[k0]]), ldc, np.array([[0.0]]), 3.0, 8.0, 1.54)[0]
+        wa = oracle.weights_2d(rows[ik], om.ze, 100)[2]
+        wb = oracle.weights_2d(rows[ik + 1], om.ze, 100)[2]
+        ta = om.weights.copy()
+        ta[ik] = ta[ik + 1] = wa
+        tb = om.weights.copy()
+        tb[ik] = tb[ik + 1] = wb
+        fa, fb = flux_with(ta), flux_with(tb)
+        f = om.evaluate(np.array([[k0]]), ldc, np.array([[0.0]]), 3.0, 8.0, 1.54)[0]
+        assert np.abs(fa - fb).max() > 1e-7               # the two rows are distinguishable
+        np.testing.assert_allclose(f, (1 - ak) * fa + ak * fb, atol=1e-15, rtol=0)
+
+
+def test_tsmodel_table_bounds(oracle):
+    """TSModel with precompute_weights: the table serves mean k in [kmin, kmax]; above
+    kmax (where the reference reads past its table) the exact weights at mean k are used,
+    as RR does, so the flux equals the exact-weight flux."""
+    ze, zm = oracle.create_z_grid(0.7, 20, 20)
+    mu = oracle.mu_nodes(zm)
+    times = np.linspace(-0.12, 0.12, 201)
+    ldp, ist = oracle.evaluate_ld('quadratic', mu, np.array([[[0.3, 0.1], [0.4, 0.2]]]))
+    dk, dg, W = oracle.weights_3d(64, 0.01, 0.1, ze, 100)
+    args = ([0.0], [3.0], [8.0], [1.54], [0.0], [0.0], [1], [0.0], ldp, ist)
+    k = np.array([[0.14, 0.16]])
+    f_tab = oracle.tsmodel_serial(times, k, *args, W, dk, 0.01, 0.1, 100, dg, ze)
+    f_exact = oracle.tsmodel_serial(times, k, *args, None, None, 0.01, 0.1, 100, None, ze)
+    np.testing.assert_array_equal(f_tab, f_exact)
+    k = np.array([[0.05, 0.052]])                     # inside the table: the lerp, close to exact
+    f_tab = oracle.tsmodel_serial(times, k, *args, W, dk, 0.01, 0.1, 100, dg, ze)
+    f_exact = oracle.tsmodel_serial(times, k, *args, None, None, 0.01, 0.1, 100, None, ze)
+    assert 0 < np.abs(f_tab - f_exact).max() < 1e-5
+
+
 def test_reference_flux_fixture_structure(oracle):
     """tests/reference_flux.npz (no generator recorded): symmetric, exactly 1
     out of transit, and column 0 is a k = 0.1 near-uniform central transit."""

@@ -70,6 +70,7 @@ class FakeExec:
         self.ws_per_pv = ws_per_pv
         self.launches = []
         self.uploads = []
+        self.retired = []
 
     def workspace_bytes(self, r):
         return self.ws_per_pv * r.npv
@@ -79,6 +80,10 @@ class FakeExec:
 
     def upload_dataset(self, ds):
         self.uploads.append(ds)
+        ds.ready = FakeEvent()
+
+    def retire_dataset(self, ds):
+        self.retired.append(ds)
 
     def launch(self, slot, reqs):
         srv = self.server_ref[0]
@@ -202,6 +207,51 @@ def test_dataset_eviction_lru_only_when_idle():
     submit(srv, d1, 1)
     srv.step()
     assert ex.uploads.count(d1) == 2
+
+
+def test_idle_datasets_are_evicted_for_request_memory():
+    """Idle resident datasets give way to a request's parameters, workspace and flux,
+    not only to another dataset; the victim's upload is retired (waited for) first."""
+    d1, d2 = dataset(npt=20000), dataset(npt=20000)
+    budget = 2 * d1.nbytes + 200_000
+    srv, ex = make_server(budget, slots=4, max_batch=1, ws_per_pv=1024)
+    for d in (d1, d2):
+        submit(srv, d, 1)
+        srv.step()
+    finish_all(ex)
+    srv.step()
+    assert d1.offset is not None and d2.offset is not None and d1.users == d2.users == 0
+    # 3 pv on d1: 3 * 20000 * 4 B of flux does not fit beside two resident datasets
+    r = submit(srv, d1, 3)
+    assert r.flux_bytes + d1.nbytes < budget
+    srv.step()
+    assert r.t_admit is not None, 'the idle dataset d2 must be evicted for the request'
+    assert d2.offset is None and d1.offset is not None and ex.retired == [d2]
+    drain(srv, ex)
+    assert r.done
+
+
+def test_run_raises_when_unreleased_results_hold_the_arena():
+    ref = [None]
+    ex = FakeExec(ref)
+    srv = RRServer(hbm_budget=700_000, slots=2, max_batch=4, precision='mixed', executor=ex, auto_release=False)
+    ref[0] = srv
+    ds = dataset(npt=1000)
+    a = submit(srv, ds, 64)
+    srv.step()
+    finish_all(ex)
+    srv.step()
+    assert a.done                                          # its flux is still held
+    b = submit(srv, ds, 64)
+    c = submit(srv, ds, 64)
+    srv.step()
+    finish_all(ex)
+    with pytest.raises(RuntimeError, match='cannot be admitted'):
+        srv.run()
+    a.release()
+    b.release()
+    drain(srv, ex)
+    assert c.done
 
 
 def test_request_validation():
