@@ -15,7 +15,12 @@ OUT = os.path.join(HERE, 'librr_hip.so')
 
 HIPCC_FLAGS = [
     '--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
-    # no packed-FP32 instructions anywhere in this library
+    # No packed-FP32 instructions (v_pk_fma/mul/add_f32) anywhere in the device
+    # code.  The device pass honours the feature; the host (x86-64) pass does not
+    # know it and prints "'-packed-fp32-ops' is not a recognized feature for this
+    # target (ignoring feature)" once per source, which says nothing about the
+    # device code.  tests/test_isa.py disassembles the built gfx950 code objects
+    # and checks that none of these instructions is in them.
     '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
     # FP32 sqrt/div to 1 ulp (the mixed path's error budget is in ppm of flux)
     '-fno-hip-fp32-correctly-rounded-divide-sqrt',
@@ -43,8 +48,13 @@ def build_library(verbose: bool = False) -> str:
     if verbose:
         print(' '.join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
+    # the host pass's note on the device-only feature (see HIPCC_FLAGS) is printed
+    # once, not once per source
     noise = "'-packed-fp32-ops' is not a recognized feature"
-    errs = '\n'.join(l for l in (res.stdout + res.stderr).splitlines() if noise not in l)
+    lines = (res.stdout + res.stderr).splitlines()
+    first = next((l for l in lines if noise in l), None)
+    errs = '\n'.join([l for l in lines if noise not in l] +
+                      ([first + '  [host pass only; the gfx950 device pass applies it]'] if first else []))
     if res.returncode != 0:
         raise RuntimeError(f'hipcc failed ({res.returncode}):\n{errs}')
     if verbose and errs.strip():

@@ -32,7 +32,7 @@ class QuadraticModel(TransitModel):
         self.interpolate = interpolate
         self.klims, self.nk, self.nz = klims, (nk if interpolate else 0), (nz if interpolate else 0)
         self._device = torch.device(device) if device is not None else None
-        self._ws = None
+        self._ws = {}   # (device, stream) -> workspace; see RoadRunnerModel._workspace
 
     @property
     def device(self) -> torch.device:
@@ -94,11 +94,14 @@ class QuadraticModel(TransitModel):
         pv = RRParams(npv=npv, kcols=k.shape[1], t0cols=t0.shape[1], pad_=0, k=ptr(k), t0=ptr(t0), p=ptr(p),
                       a=ptr(a), inc=ptr(inc), e=ptr(e), w=ptr(w))
         nbytes = max(int(lib.ma_workspace_bytes(npv)), 256)
-        if self._ws is None or self._ws.numel() < nbytes or self._ws.device != device:
-            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        key = (str(device), stream_handle(device))
+        ws = self._ws.get(key)
+        if ws is None or ws.numel() < nbytes:
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
+            self._ws[key] = ws
         flux = torch.empty((npv, self.npt), dtype=torch.float64, device=device)
-        check(lib.ma_evaluate(_lib.ctypes.byref(lc), _lib.ctypes.byref(pv), ptr(ldc), ptr(self._ws),
-                              self._ws.numel(), ptr(flux), stream_handle(device)))
+        check(lib.ma_evaluate(_lib.ctypes.byref(lc), _lib.ctypes.byref(pv), ptr(ldc), ptr(ws),
+                              ws.numel(), ptr(flux), stream_handle(device)))
         return flux
 
     def __call__(self, *args, **kwargs):

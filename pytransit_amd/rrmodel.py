@@ -187,7 +187,13 @@ class RoadRunnerModel(TransitModel):
 
     # ---------------------------------------------------------------------------------
     def _workspace(self, device, nbytes: int) -> torch.Tensor:
-        key = str(device)
+        """The per-pv workspace of torch's current stream on ``device``.  One buffer
+        per (device, stream): calls issued on different streams may overlap on the
+        device, and the setup of one would overwrite the records the flux kernel of
+        the other is still reading.  A buffer is allocated on, and only ever used
+        by, its own stream, so the caching allocator can reuse a replaced buffer's
+        memory only after that stream's earlier kernels."""
+        key = (str(device), stream_handle(device))
         ws = self._ws.get(key)
         if ws is None or ws.numel() < nbytes:
             ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
