@@ -11,13 +11,19 @@ BJD 2459000.0), 10 supersamples over a 30-min exposure, power-2 limb darkening.
 precision path (FP64 epoch reduction and setup, FP32 flux).  The same workload is
 timed in plain FP64 and the maximum |F_mixed - F_fp64| over every point is
 reported in ppm.  ``roofline`` is for the flux kernel alone, timed with HIP events
-on the stream it is launched on; ``cpu_baseline`` is the C/OpenMP FP64
-restatement of the reference algorithm on this host's cores.
+on the stream it is launched on: algorithmic HBM bytes per launch against 8 TB/s,
+and (``roofline.flop``) algorithmic FLOPs per launch, counted exactly by the
+oracle on this workload (profiles/rr_flux_rr_bjd_flop_count.md), against the
+78.6 TFLOP/s vector peak; ``bound`` names the larger fraction.  ``cpu_baseline``
+is the C/OpenMP FP64 restatement of the reference algorithm on this host's cores.
 
 Further legs, reported beside the headline in the same JSON line:
 * ``ts_many_pb``: TSModel, 1,024 pv x 5,000 times x 512 passbands (mixed and FP64);
 * ``gradient``: fused log-likelihood with its adjoint gradient, 2,048 pv x 10,000
   times (mixed and FP64);
+* ``c0``: the CPU-runnable case, 64 pv x 2,000 times, quadratic, nss=1 (FP64,
+  mixed, and the C/OpenMP restatement on the host);
+* ``with_d2h``: the headline workload with ``copy=True`` (flux copied to host);
 * ``serving``: a Poisson stream of 1000 fit requests through the request server,
   batched (4 slots, up to 16 requests and 2^28 points per launch) and unbatched
   (one at a time), with throughput and p50/p99 latency; ``capacity`` offers the
@@ -59,6 +65,11 @@ def parse():
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
                             'rr_flux_mixed_rr_bjd_hbm_traffic.md')
+FLOP_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                         'rr_flux_rr_bjd_flop_count.md')
+# MI355X vector peaks without packed-FP32 instructions (the library is built
+# without them): 256 CUs x 128 FLOP/clock x 2.4 GHz, for FP32 and FP64 alike.
+VALU_PEAK_TFLOPS = 78.6
 
 
 def measured_traffic():
@@ -68,6 +79,18 @@ def measured_traffic():
         with open(TRAFFIC_FILE) as f:
             m = re.search(r'traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)', f.read())
         return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def flop_count():
+    """Algorithmic FLOPs of one flux evaluation of the rr_bjd workload, counted
+    with the oracle's window and geometry (profiles/rr_flux_rr_bjd_flop_count.md,
+    written by ``python -m oracle.flop_count``), or None."""
+    try:
+        with open(FLOP_FILE) as f:
+            m = re.search(r'flops_per_launch\s*=\s*([0-9]+)', f.read())
+        return int(m.group(1)) if m else None
     except OSError:
         return None
 
@@ -253,6 +276,52 @@ def grad_leg(torch, dev, steps, warmup):
             "max_abs_lnl_diff_mixed_vs_fp64": float((res['mixed'][1][0] - res['fp64'][1][0]).abs().max())}
 
 
+def c0_leg(torch, dev, steps, warmup, cpu):
+    """The CPU-runnable case: 64 pv x 2,000 times, quadratic LD, no supersampling,
+    FP64 and mixed, on the device and (``cpu``) with the C/OpenMP restatement.
+    It writes 1 MB, so its time is launch latency, not a roofline position."""
+    from pytransit_amd import RoadRunnerModel, workloads
+    wl = workloads.rr_small()
+    npts = wl.k.shape[0] * wl.times.size
+    args_dev = tuple(torch.as_tensor(x, device=dev) for x in (wl.k, wl.ldc, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w))
+    out = {"workload": f"rr_small: RoadRunnerModel('quadratic') evaluate, npv={wl.k.shape[0]} x npt={wl.times.size}, "
+                       f"nss=1", "points": npts}
+    for prec in ('fp64', 'mixed'):
+        tm = RoadRunnerModel(wl.law, precision=prec, device=dev)
+        tm.set_data(wl.times)
+        sec, _ = _timed(torch, lambda: tm._evaluate_device(*args_dev), steps, warmup)
+        out[prec + "_points_per_s"] = npts / sec
+        out[prec + "_ms"] = sec * 1e3
+    if cpu:
+        from oracle import rr_cpu
+        nthreads = min(len(os.sched_getaffinity(0)), int(os.environ.get('OMP_NUM_THREADS', 1 << 30)))
+        cm = rr_cpu.CpuRR('quadratic')
+        cm.set_data(wl.times)
+        a = (wl.k, wl.ldc, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w)
+        cm.evaluate(*a, nthreads=nthreads)
+        ts = []
+        for _ in range(20):
+            t = time.perf_counter()
+            cm.evaluate(*a, nthreads=nthreads)
+            ts.append(time.perf_counter() - t)
+        out["cpu_points_per_s"] = npts / statistics.median(ts)
+        out["cpu_cores"] = nthreads
+    return out
+
+
+def d2h_rate(torch, tm, args_dev, steps):
+    """points/s of evaluate(copy=True): the device result copied to host memory
+    (numpy), i.e. the PCIe-inclusive rate of the reference's call semantics."""
+    npv, npt = args_dev[0].shape[0], tm.npt
+    tm.evaluate(*args_dev, copy=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        f = tm.evaluate(*args_dev, copy=True)
+    sec = (time.perf_counter() - t) / steps
+    return {"points_per_s": npv * npt / sec, "ms": sec * 1e3, "bytes": int(f.nbytes)}
+
+
 def workloads_mod():
     from pytransit_amd import workloads
     return workloads
@@ -309,11 +378,13 @@ def main():
     args_dev = tuple(torch.as_tensor(x, device=dev) for x in (wl.k, wl.ldc, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w))
 
     res = {}
+    d2h = {}
     for prec in ('mixed', 'fp64'):
         tm = RoadRunnerModel(wl.law, precision=prec, device=dev)
         tm.set_data(wl.times, nsamples=wl.nsamples, exptimes=wl.exptimes)
         step_s, kern_ms, out = time_path(tm, args_dev, args.steps, args.warmup)
         res[prec] = dict(step_s=step_s, kern_ms=kern_ms, out=out, ng=tm.ng)
+        d2h[prec] = d2h_rate(torch, tm, args_dev, max(3, args.steps // 10))
 
     err = (res['mixed']['out'].double() - res['fp64']['out']).abs()
     max_err_ppm = float(err.max()) * 1e6
@@ -325,6 +396,17 @@ def main():
     kbytes = flux_kernel_bytes(npv, npt, res['mixed']['ng'], 4)
     achieved = kbytes / (res['mixed']['kern_ms'] * 1e-3) / 1e9
     traffic = measured_traffic()
+    flops = flop_count()
+    flop = None
+    if flops is not None:
+        tf = {prec: flops / (res[prec]['kern_ms'] * 1e-3) / 1e12 for prec in res}
+        flop = {"achieved": tf['mixed'], "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tf['mixed'] / VALU_PEAK_TFLOPS, "flops_per_launch": flops,
+                "fp64_achieved": tf['fp64'], "fp64_frac": tf['fp64'] / VALU_PEAK_TFLOPS,
+                "peak_note": "vector FP32 without packed instructions, and FP64: 78.6 TFLOP/s",
+                "count_source": os.path.relpath(FLOP_FILE)}
+    hbm_frac = achieved / HBM_PEAK_GBS
+    flop_frac = flop['frac'] if flop else 0.0
     line = {
         "metric": METRIC,
         "value": mixed_pps,
@@ -346,13 +428,18 @@ def main():
         "mixed_over_fp64": mixed_pps / fp64_pps,
         "max_abs_err_ppm_mixed_vs_fp64": max_err_ppm,
         "nan_pattern_equal": both_nan,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "roofline": {"bound": "hbm" if hbm_frac >= flop_frac else "valu", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_frac, "traffic": traffic,
                      "kernel": "rr_flux_kernel<mixed>", "kernel_ms": res['mixed']['kern_ms'],
                      "algorithmic_bytes": kbytes,
                      "fp64_kernel_ms": res['fp64']['kern_ms'],
-                     "traffic_source": os.path.relpath(TRAFFIC_FILE) if traffic is not None else None},
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE) if traffic is not None else None,
+                     "flop": flop},
+        "with_d2h": {"mixed_points_per_s": d2h['mixed']['points_per_s'], "mixed_ms": d2h['mixed']['ms'],
+                     "fp64_points_per_s": d2h['fp64']['points_per_s'], "fp64_ms": d2h['fp64']['ms'],
+                     "note": "evaluate(copy=True): the flux copied to a numpy array each step"},
     }
+    line["c0"] = c0_leg(torch, dev, args.steps, args.warmup, not args.no_cpu)
     if not args.no_extra:
         line["ts_many_pb"] = ts_leg(torch, dev, max(3, args.steps // 4), 1)
         line["gradient"] = grad_leg(torch, dev, max(3, args.steps // 4), 1)

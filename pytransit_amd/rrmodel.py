@@ -306,11 +306,9 @@ class RoadRunnerModel(TransitModel):
         flux = self._evaluate_device(k, ldc, t0, p, a, i, e, w).squeeze()
         return flux.cpu().numpy() if copy else flux
 
-    def _evaluate_device(self, k, ldc, t0, p, a, i, e, w, flux_events=None):
-        """All three stages on torch's current stream; returns the [npv, npt] device
-        flux.  ``flux_events`` = (start, end) torch.cuda.Event pair recorded around
-        the flux kernel alone."""
-        lib = _lib.load()
+    def _descriptors(self, k, ldc, t0, p, a, i, e, w):
+        """The C-ABI descriptors of one evaluate call: (npv, RRTable, RRLcData,
+        RRParams, RRLd, keep), where ``keep`` holds the device tensors they point to."""
         device = self.device
         if self.time is None:
             raise ValueError('set_data() must be called before evaluate()')
@@ -331,6 +329,16 @@ class RoadRunnerModel(TransitModel):
                       nsamples=ptr(d['nsamples']), exptimes=ptr(d['exptimes']))
         prm = RRParams(npv=npv, kcols=k.shape[1], t0cols=t0.shape[1], pad_=0, k=ptr(k), t0=ptr(t0), p=ptr(p),
                        a=ptr(a), inc=ptr(inc), e=ptr(e), w=ptr(w))
+        return npv, tab, lc, prm, ld, (k, t0, p, a, inc, e, w, _ld_keep, _keep)
+
+    def _evaluate_device(self, k, ldc, t0, p, a, i, e, w, flux_events=None):
+        """All three stages on torch's current stream; returns the [npv, npt] device
+        flux.  ``flux_events`` = (start, end) torch.cuda.Event pair recorded around
+        the flux kernel alone."""
+        lib = _lib.load()
+        device = self.device
+        npv, tab, lc, prm, ld, _keep = self._descriptors(k, ldc, t0, p, a, i, e, w)
+        npb = self.npb
         prec = _PRECISIONS[self.precision]
         dtype = torch.float64 if prec == _lib.RR_FP64 else torch.float32
         flux = torch.empty((npv, self.npt), dtype=dtype, device=device)

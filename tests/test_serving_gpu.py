@@ -96,3 +96,31 @@ def test_batched_kernels_against_oracle(gpu, oracle):
             err = np.abs(got - ref).max()
             say(f'  {prec} {rq["ld"]:10s} npv={r.npv} npt={r.dataset.npt} npb={r.dataset.npb} max err {err:.2e}')
             assert err <= tol
+
+
+@pytest.mark.parametrize('law', ['power-2', 'nonlinear'])
+def test_top_of_range_request_against_oracle(gpu, oracle, law):
+    # the request stream's top of range: 64 passbands (64 contiguous light curves)
+    # and 120 000 times at 2-min cadence, nss = 10, on a 12-pv slice.  power-2 takes
+    # the setup's 8-passband group path, the nonlinear law its 64-passband basis
+    # block; the flux kernel stages several light curves per tile.
+    npb, npt, npv = 64, 120_000, 12
+    say(f'case: serving top of range, {law}, {npv} pv x {npt} t x {npb} pb, nss=10')
+    rng = np.random.default_rng(21 if law == 'power-2' else 22)
+    times = 2459000.0 + np.arange(npt) * (2.0 / 1440.0)
+    ds_kw = dict(time=times, lcids=(np.arange(npt) * npb) // npt, pbids=np.arange(npb), nsamples=10,
+                 exptimes=30.0 / 1440.0)
+    rq = workloads.serving_request(rng, npb, (npv, npv))
+    rq.update(ld=law, ldc=workloads._ld_draw(rng, law, npv, npb))
+    om = oracle.OracleRR(law)
+    om.set_data(**ds_kw)
+    ref = om.evaluate(rq['k'], rq['ldc'], rq['t0'][:, None], rq['p'], rq['a'], rq['i'])
+    assert (ref < 1).sum() > 10000
+    for prec, tol in (('fp64', 1e-9), ('mixed', 1e-6)):
+        srv = RRServer(hbm_budget=1 << 30, slots=2, max_batch=16, precision=prec)
+        r = srv.submit(Dataset(**ds_kw), rq['k'], rq['ldc'], rq['t0'], rq['p'], rq['a'], rq['i'], ld=law)
+        srv.run(timeout=60)
+        got = r.result().astype(np.float64)
+        err = np.abs(got - ref).max()
+        say(f'  {prec}: max |dF| = {err:.2e}')
+        assert err <= tol

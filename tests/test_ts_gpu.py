@@ -128,3 +128,21 @@ def test_ts_wide_g_grid_and_two_samples(gpu, oracle, prec, tol, nss):
     say(f'  max |dF| = {err:.3e}')
     assert (fo < 1).sum() > 10000
     assert err <= tol
+
+
+@pytest.mark.parametrize('prec,tol', [('fp64', FP64_TOL), ('mixed', MIXED_TOL)])
+def test_ts_full_passband_count(gpu, oracle, prec, tol):
+    # the many-passband workload's full shape on a 2-pv slice: 512 passbands x 5000
+    # times on 128 mu nodes, tabulated profiles, exact weights at mean(k).  512 rows
+    # take the MFMA contraction's full row-tile loop and the row-form flux kernel.
+    say(f'case: TS 2 pv x 512 pb x 5000 t, 128 mu nodes, tabulated LD, {prec}')
+    wl, ze, mu = ts_case(oracle, npv=2, npt=5000, npb=512, seed=2)
+    fo = oracle_ts(oracle, wl, ze)
+    tm = TSModel(TabulatedLDModel(wl.ldp, wl.istar), nzin=64, nzlimb=64, precision=prec)
+    tm.set_data(wl.times)
+    fg = np.asarray(tm.evaluate(wl.k, None, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w), float)
+    assert fg.shape == fo.shape == (2, 512, 5000)
+    err = np.max(np.abs(fg - fo))
+    say(f'  max |dF| = {err:.3e}, in-transit points {(fo < 1).sum()}')
+    assert (fo < 1).sum() > 1_000_000
+    assert err <= tol
