@@ -25,10 +25,11 @@ Further legs, reported beside the headline in the same JSON line:
   mixed, and the C/OpenMP restatement on the host);
 * ``with_d2h``: the headline workload with ``copy=True`` (flux copied to host);
 * ``serving``: a Poisson stream of 1000 fit requests through the request server,
-  batched (4 slots, up to 16 requests and 2^28 points per launch) and unbatched
-  (one at a time), with throughput and p50/p99 latency; ``capacity`` offers the
-  same mix at twice the rate, which neither server keeps up with, and reports
-  what each serves.
+  batched (4 slots, up to 16 requests and 2^28 points per launch), batched with
+  the launches replayed from captured HIP graphs, and unbatched (one at a time),
+  with throughput and p50/p99 latency of the second of two passes over the
+  stream (the first warms each server); ``capacity`` offers the same mix at
+  twice the rate and reports what each serves.
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 3] [--no-cpu] [--no-serve] [--no-extra]
 """
@@ -137,61 +138,68 @@ def time_path(tm, args_dev, steps, warmup):
     return wall / steps, kern_ms, out
 
 
-def serve_stream(torch, dev, st, slots, max_batch, budget, max_points=None):
-    """Replay the request stream in real time through one server; returns the
-    per-request latencies (s), the points served and the wall time from the
-    first arrival to the last completion."""
+def serve_stream(torch, dev, st, slots, max_batch, budget, max_points=None, graphs=False):
+    """Replay the request stream in real time through one server, twice: the first
+    pass warms the server (every light-curve set resident, kernels loaded and,
+    with ``graphs``, the launch graphs of the stream's shapes captured), the second
+    is measured.  Returns the second pass's per-request latencies (s), the points
+    served, the wall time from the first arrival to the last completion, and the
+    graphs captured during it."""
     from pytransit_amd.serving import Dataset, RRServer
     dsets = [Dataset(**d) for d in st.datasets]
     srv = RRServer(hbm_budget=budget, slots=slots, max_batch=max_batch, precision='mixed', device=dev,
-                   auto_release=True, max_batch_points=max_points)
-    # warm-up: every light-curve set resident, kernels loaded
-    for j, d in enumerate(dsets):
-        rq = next((r for r in st.requests if r['dataset'] == j), None)
-        if rq is not None:
-            srv.submit(d, rq['k'][:8], rq['ldc'][:8], rq['t0'][:8], rq['p'][:8], rq['a'][:8], rq['i'][:8],
-                       ld=rq['ld'])
-    srv.run(timeout=120)
-    torch.cuda.synchronize()
-    srv.completed.clear()
+                   auto_release=True, max_batch_points=max_points, graphs=graphs)
     reqs = st.requests
     n = len(reqs)
-    t_start = time.perf_counter()
-    i = 0
-    while i < n or not srv.idle:
-        now = time.perf_counter() - t_start
-        while i < n and reqs[i]['arrival'] <= now:
-            rq = reqs[i]
-            srv.submit(dsets[rq['dataset']], rq['k'], rq['ldc'], rq['t0'], rq['p'], rq['a'], rq['i'], ld=rq['ld'],
-                       arrival=t_start + rq['arrival'])
-            i += 1
-        srv.step()
+    for npass in range(2):
+        torch.cuda.synchronize()
+        srv.completed.clear()
+        captures0 = srv._exec.graph_captures
+        t_start = time.perf_counter()
+        i = 0
+        while i < n or not srv.idle:
+            now = time.perf_counter() - t_start
+            while i < n and reqs[i]['arrival'] <= now:
+                rq = reqs[i]
+                srv.submit(dsets[rq['dataset']], rq['k'], rq['ldc'], rq['t0'], rq['p'], rq['a'], rq['i'],
+                           ld=rq['ld'], arrival=t_start + rq['arrival'])
+                i += 1
+            srv.step()
     done = srv.completed
     lat = np.array([r.latency for r in done])
     wall = max(r.t_done for r in done) - t_start
     points = sum(r.points for r in done)
-    return dict(latency=lat, points=points, wall=wall, batch=np.mean([r.batch_size for r in done]))
+    return dict(latency=lat, points=points, wall=wall, batch=np.mean([r.batch_size for r in done]),
+                captures=srv._exec.graph_captures - captures0, graphs=srv._exec.graph_captures)
 
 
 def serving_leg(torch, dev, nreq, rate):
     st = workloads_mod().serving_stream(n_requests=nreq, rate=rate)
     budget = 48 << 30
     out = {}
-    # batched: up to 16 requests and 2^28 points (~0.6 ms of device time) per launch
-    for name, slots, mb, mp in (('batched', 4, 16, 1 << 28), ('unbatched', 1, 1, None)):
-        r = serve_stream(torch, dev, st, slots, mb, budget, mp)
+    # batched: up to 16 requests and 2^28 points (~0.6 ms of device time) per launch;
+    # batched_graphs: the same with the launches replayed from captured graphs
+    servers = (('batched', 4, 16, 1 << 28, False), ('batched_graphs', 4, 16, 1 << 28, True),
+               ('unbatched', 1, 1, None, False))
+    for name, slots, mb, mp, gr in servers:
+        r = serve_stream(torch, dev, st, slots, mb, budget, mp, gr)
         out[name] = {"throughput_points_per_s": r['points'] / r['wall'], "requests_per_s": nreq / r['wall'],
                      "p50_ms": float(np.percentile(r['latency'], 50) * 1e3),
                      "p99_ms": float(np.percentile(r['latency'], 99) * 1e3),
                      "wall_s": r['wall'], "mean_batch": float(r['batch']), "slots": slots, "max_batch": mb,
                      "max_batch_points": mp}
-    # capacity: the same mix offered at twice the rate, which neither server keeps up with
+        if gr:
+            out[name]["graphs"] = r['graphs']
+            out[name]["graphs_captured_in_measured_pass"] = r['captures']
+    # capacity: the same mix offered at twice the rate
     st2 = workloads_mod().serving_stream(n_requests=nreq, rate=2 * rate)
     cap = {"offered_requests_per_s": 2 * rate}
-    for name, slots, mb, mp in (('batched', 4, 16, 1 << 28), ('unbatched', 1, 1, None)):
-        r = serve_stream(torch, dev, st2, slots, mb, budget, mp)
+    for name, slots, mb, mp, gr in servers:
+        r = serve_stream(torch, dev, st2, slots, mb, budget, mp, gr)
         cap[name + "_points_per_s"] = r['points'] / r['wall']
         cap[name + "_requests_per_s"] = nreq / r['wall']
+        cap[name + "_p50_ms"] = float(np.percentile(r['latency'], 50) * 1e3)
+        cap[name + "_p99_ms"] = float(np.percentile(r['latency'], 99) * 1e3)
     out["capacity"] = cap
     out["workload"] = (f"{nreq} fit requests, Poisson arrivals at {rate:g}/s; light-curve sets of log-uniform "
                        f"1k-200k points in 1-64 passbands (24-set pool, resident), log-uniform 32-8192 pv, "

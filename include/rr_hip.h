@@ -196,6 +196,26 @@ int rr_max_batch(void);
 int rr_evaluate_batch(int precision, const rr_table* tab, int nreq, const rr_request* reqs,
                       void* stream);
 
+/* The batch of rr_evaluate_batch split into a plan on the host and a launch that
+ * reads the plan from device memory, so the launch can be captured in a HIP graph
+ * once and replayed for every later batch whose shape it covers.  rr_batch_plan
+ * validates the requests and writes the batch's descriptor (rr_batch_desc_bytes()
+ * bytes, plain data) into `desc` (host memory) and its launch shape into *shape.
+ * The caller copies the descriptor to device memory and calls rr_batch_launch with
+ * that device copy and a shape covering the plan's: the same setup_multi, and
+ * every block count and LDS size at least the plan's (blocks beyond the plan's
+ * exit at once).  The results equal rr_evaluate_batch's.  No counterpart in the
+ * reference; these are the replayable launch of the request server. */
+typedef struct rr_batch_shape {
+    int32_t setup_blocks, setup_multi, setup_lds;
+    int32_t flux_blocks[2];   /* [0] general light-curve layout, [1] contiguous sorted */
+    int32_t flux_lds[2];
+} rr_batch_shape;
+size_t rr_batch_desc_bytes(void);
+int rr_batch_plan(int precision, const rr_table* tab, int nreq, const rr_request* reqs, void* desc,
+                  rr_batch_shape* shape);
+int rr_batch_launch(int precision, const void* desc_dev, const rr_batch_shape* launch, void* stream);
+
 /* Orbit boundary alone: Taylor coefficients coeffs[npv, 2, 5] (row 0 x, row 1 y,
  * coefficient n = d^n/dt^n / n!) and contact window win[npv, 2] = (t1, t4) for
  * radius ratio k0[npv].  Replaces meepmeep's solve2d + bounding_box as called at
@@ -238,6 +258,39 @@ int rr_lnlike(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_
 size_t ma_workspace_bytes(int npv);
 int ma_evaluate(const rr_lcdata* lc, const rr_params* pv, const double* ldc, void* workspace,
                 size_t workspace_bytes, double* flux, void* stream);
+
+/* Secondary eclipse of a uniform planet disk: flux[npv, npt] (FP64) is the planet
+ * disk area left visible by the star, pi k^2 outside the eclipse window.  k is
+ * [npv] (kcols = 1), t0 [npv, t0cols] indexed by lc->epids (EclipseModel passes
+ * the light curves' passband indices there, new_eclipse_model.py:70), rstar[npv]
+ * the stellar radius in solar radii (light travel time).  The orbit is the
+ * in-repo Taylor series about mid-eclipse (orbits/taylor_z.py:106-185) at the
+ * conjunction phase (orbits/orbits_py.py:544-555); the contact window gets
+ * 0.003 d + the exposure time.  Invalid pv (a <= 1, a NaN, e < 0) give NaN rows.
+ * Replaces eclipse_model (pytransit/models/roadrunner/model_eclipse.py:11-81). */
+size_t ec_workspace_bytes(int npv);
+int eclipse_evaluate(const rr_lcdata* lc, const rr_params* pv, const double* rstar, void* workspace,
+                     size_t workspace_bytes, double* flux, void* stream);
+
+/* Eclipse spectroscopy: flux[npv, npb, npt] (FP64) = 1 - (f A / pi) / (1 + f k^2)
+ * averaged over the exposure for flux ratios fratio[npv, npb], 1 outside the
+ * window (0.0015 d + the exposure time).  t0 is [npv] (t0cols = 1); nsamples[0]
+ * and exptimes[0] apply to every time.  Replaces esmodel
+ * (pytransit/models/roadrunner/model_ecspec.py:13-65) as called by
+ * EclipseSpectroscopyModel.evaluate (esmodel.py:46-88). */
+int es_evaluate(const rr_lcdata* lc, const rr_params* pv, const double* rstar, const double* fratio, int npb,
+                void* workspace, size_t workspace_bytes, double* flux, void* stream);
+
+/* Oblate planet: the RoadRunner model (FP64) with the planet's sky projection an
+ * ellipse of semi-axes k and k (1 - f[pv]) rotated by alpha[pv], its area inside
+ * the stellar disk summed over nlines scanlines (2 <= nlines <= 2048).  The
+ * workspace is rr_evaluate's (rr_workspace_bytes); the per-pv setup is rr_setup's.
+ * Replaces op_full_serial (pytransit/models/roadrunner/opmodel_full.py:40-134) and
+ * the scanline area of ecintersection.py:93-207, as called by
+ * OblatePlanetModel.evaluate (opmodel.py:50-111). */
+int op_evaluate(const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const rr_ld* ld, const double* f,
+                const double* alpha, int nlines, void* workspace, size_t workspace_bytes, double* flux,
+                void* stream);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,11 @@ class RRRequest(ctypes.Structure):
                 ('workspace_bytes', c_size_t), ('flux', c_void_p)]
 
 
+class RRBatchShape(ctypes.Structure):
+    _fields_ = [('setup_blocks', c_int32), ('setup_multi', c_int32), ('setup_lds', c_int32),
+                ('flux_blocks', c_int32 * 2), ('flux_lds', c_int32 * 2)]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     'rr_last_error': (ctypes.c_char_p, []),
@@ -87,6 +92,17 @@ _SIGNATURES = {
     'ma_workspace_bytes': (c_size_t, [c_int]),
     'ma_evaluate': (c_int, [POINTER(RRLcData), POINTER(RRParams), c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
     'rr_evaluate_batch': (c_int, [c_int, POINTER(RRTable), c_int, POINTER(RRRequest), c_void_p]),
+    'rr_batch_desc_bytes': (c_size_t, []),
+    'rr_batch_plan': (c_int, [c_int, POINTER(RRTable), c_int, POINTER(RRRequest), c_void_p,
+                              POINTER(RRBatchShape)]),
+    'rr_batch_launch': (c_int, [c_int, c_void_p, POINTER(RRBatchShape), c_void_p]),
+    'ec_workspace_bytes': (c_size_t, [c_int]),
+    'eclipse_evaluate': (c_int, [POINTER(RRLcData), POINTER(RRParams), c_void_p, c_void_p, c_size_t, c_void_p,
+                                 c_void_p]),
+    'es_evaluate': (c_int, [POINTER(RRLcData), POINTER(RRParams), c_void_p, c_void_p, c_int, c_void_p, c_size_t,
+                            c_void_p, c_void_p]),
+    'op_evaluate': (c_int, [POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd), c_void_p,
+                            c_void_p, c_int, c_void_p, c_size_t, c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -10,7 +10,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
-SOURCES = ['rr_kernels.hip', 'ts_kernels.hip', 'adjoint_kernels.hip', 'ma_kernels.hip']
+SOURCES = ['rr_kernels.hip', 'ts_kernels.hip', 'adjoint_kernels.hip', 'ma_kernels.hip', 'ec_kernels.hip']
 OUT = os.path.join(HERE, 'librr_hip.so')
 
 HIPCC_FLAGS = [

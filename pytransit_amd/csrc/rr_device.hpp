@@ -218,12 +218,16 @@ __device__ __forceinline__ double taylor_z_wrapped(double t, const double* c) {
 }
 
 // Contact time by bisection, tol 1e-6 d and at most 100 steps; taylor_z.py:298-328.
-// point 1 or 4 (first/fourth contact, z = 1+k).
-__device__ __forceinline__ double find_contact_point(double k, int point, const double* c) {
+// point 1 or 4 (first/fourth contact, z = 1+k).  The first bracket is [0, s 2/v]
+// with s = -1 before mid-transit and +1 after; the reference takes v = vx, the
+// sky-plane x velocity (positive at a transit).  The eclipse path passes |vx|:
+// behind the star the planet moves in -x, and v = vx would put first contact after
+// mid-eclipse.
+__device__ __forceinline__ double contact_bisect(double k, int point, const double* c, double v) {
     const double s = (point == 1 || point == 2 || point == 12) ? -1.0 : 1.0;
     const double zt = (point == 1 || point == 4) ? 1.0 + k : ((point == 2 || point == 3) ? 1.0 - k : 1.0);
     double t0 = 0.0;
-    double t2 = s * 2.0 / c[1];
+    double t2 = s * 2.0 / v;
     double t1 = 0.5 * t2;
     double z0 = taylor_z_wrapped(t0, c) - zt;
     double z1 = taylor_z_wrapped(t1, c) - zt;
@@ -242,6 +246,10 @@ __device__ __forceinline__ double find_contact_point(double k, int point, const 
         ++i;
     }
     return t1;
+}
+
+__device__ __forceinline__ double find_contact_point(double k, int point, const double* c) {
+    return contact_bisect(k, point, c, c[1]);
 }
 
 // ---- Epoch reduction (model_full.py:88-89) -------------------------------------------

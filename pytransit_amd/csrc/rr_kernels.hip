@@ -818,6 +818,19 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) rr
     setup_body<kMulti>(b.a[i], blockIdx.x - b.start[i]);
 }
 
+// The same with the batch in device memory, for launches that are captured once
+// and replayed with new contents: the grid may exceed the batch's blocks
+// (start[n]); the blocks past them exit at once.
+template <bool kMulti>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
+rr_setup_batch_dev_kernel(const SetupBatch* __restrict__ b) {
+    const int n = b->n;
+    if (n == 0 || (int)blockIdx.x >= b->start[n]) return;
+    const int i = batch_item(b->start, n, blockIdx.x);
+    if (!kMulti && b->a[i].npb > 1) return;   // a launch shape that does not fit its batch (caller error)
+    setup_body<kMulti>(b->a[i], blockIdx.x - b->start[i]);
+}
+
 // ------------------------------------------------------------------------------------
 // Flux; model_full.py:75-99.
 //
@@ -1578,6 +1591,26 @@ __global__ void __launch_bounds__(kThreads) rr_flux_batch_kernel(FluxBatch b) {
     flux_body<kMixed, kFast>(b.a[i], bid - b.start[i]);
 }
 
+// Device-memory batch (see rr_setup_batch_dev_kernel).  The XCD remap is over the
+// batch's own blocks, so a grid padded past them still spreads the work over all
+// XCDs; block b < start[n] is on XCD b % 8 as in the unpadded launch.
+template <bool kMixed, bool kFast>
+__global__ void __launch_bounds__(kThreads) rr_flux_batch_dev_kernel(const FluxBatch* __restrict__ b) {
+    const int n = b->n;
+    if (n == 0) return;
+    const int total = b->start[n];
+    if ((int)blockIdx.x >= total) return;
+    const int bid = xcd_remap(blockIdx.x, total);
+    const int i = batch_item(b->start, n, bid);
+    flux_body<kMixed, kFast>(b->a[i], bid - b->start[i]);
+}
+
+// A batch as the replayable launch reads it from device memory.
+struct BatchDesc {
+    SetupBatch setup;
+    FluxBatch flux[2];   // [0] general, [1] contiguous sorted light curves
+};
+
 static_assert(sizeof(SetupBatch) <= 4096 && sizeof(FluxBatch) <= 4096, "kernel arguments above 4 KB");
 
 }  // namespace rr
@@ -1870,23 +1903,26 @@ int rr_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
 
 int rr_max_batch(void) { return kMaxBatch; }
 
-int rr_evaluate_batch(int precision, const rr_table* tab, int nreq, const rr_request* reqs, void* stream) {
+// Validates every request of a batch and fills its setup and flux batches, grid
+// sizes and dynamic LDS bytes.
+static int plan_batch(const char* who0, int precision, const rr_table* tab, int nreq, const rr_request* reqs,
+                      BatchDesc& bd, rr_batch_shape& sh) {
     if (nreq < 0 || nreq > kMaxBatch)
-        return set_error(RR_EINVAL, "rr_evaluate_batch: %d requests, at most %d per call", nreq, kMaxBatch);
-    if (nreq > 0 && !reqs) return set_error(RR_EINVAL, "rr_evaluate_batch: null request array");
-    SetupBatch sb;
-    FluxBatch fb[2];  // [0] general, [1] contiguous sorted light curves
+        return set_error(RR_EINVAL, "%s: %d requests, at most %d per call", who0, nreq, kMaxBatch);
+    if (nreq > 0 && !reqs) return set_error(RR_EINVAL, "%s: null request array", who0);
+    SetupBatch& sb = bd.setup;
+    FluxBatch* fb = bd.flux;
     size_t lds[2] = {0, 0}, slds = 0;
     bool multi = false;
     long long fblocks[2] = {0, 0};
     long long sblocks = 0;
     sb.n = 0;
     fb[0].n = fb[1].n = 0;
+    sb.start[0] = fb[0].start[0] = fb[1].start[0] = 0;
     char who[48];
-    // validate every request before launching anything
     for (int r = 0; r < nreq; ++r) {
         const rr_request& q = reqs[r];
-        snprintf(who, sizeof(who), "rr_evaluate_batch[%d]", r);
+        snprintf(who, sizeof(who), "%s[%d]", who0, r);
         SetupArgs sa;
         int nb = 0;
         int rc = prepare_setup(who, precision, tab, &q.lc, &q.pv, &q.ld, q.workspace, q.workspace_bytes, sa, nb);
@@ -1910,27 +1946,100 @@ int rr_evaluate_batch(int precision, const rr_table* tab, int nreq, const rr_req
         lds[fast ? 1 : 0] = l > lds[fast ? 1 : 0] ? l : lds[fast ? 1 : 0];
     }
     if (sblocks > 0x7fffffffLL || fblocks[0] > 0x7fffffffLL || fblocks[1] > 0x7fffffffLL)
-        return set_error(RR_ERANGE, "rr_evaluate_batch: grid too large");
-    if (sb.n == 0) return RR_OK;
-    hipStream_t s = (hipStream_t)stream;
+        return set_error(RR_ERANGE, "%s: grid too large", who0);
     sb.start[sb.n] = (int)sblocks;
-    if (multi) hipLaunchKernelGGL(rr_setup_batch_kernel<true>, dim3((unsigned)sblocks), dim3(64), slds, s, sb);
-    else hipLaunchKernelGGL(rr_setup_batch_kernel<false>, dim3((unsigned)sblocks), dim3(64), slds, s, sb);
-    int rc = check_launch("rr_setup_batch_kernel");
+    fb[0].start[fb[0].n] = (int)fblocks[0];
+    fb[1].start[fb[1].n] = (int)fblocks[1];
+    sh.setup_blocks = (int32_t)sblocks;
+    sh.setup_multi = multi ? 1 : 0;
+    sh.setup_lds = (int32_t)slds;
+    for (int v = 0; v < 2; ++v) {
+        sh.flux_blocks[v] = (int32_t)fblocks[v];
+        sh.flux_lds[v] = (int32_t)lds[v];
+    }
+    return RR_OK;
+}
+
+int rr_evaluate_batch(int precision, const rr_table* tab, int nreq, const rr_request* reqs, void* stream) {
+    BatchDesc bd;
+    rr_batch_shape sh;
+    int rc = plan_batch("rr_evaluate_batch", precision, tab, nreq, reqs, bd, sh);
+    if (rc || bd.setup.n == 0) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const SetupBatch& sb = bd.setup;
+    if (sh.setup_multi)
+        hipLaunchKernelGGL(rr_setup_batch_kernel<true>, dim3((unsigned)sh.setup_blocks), dim3(64), sh.setup_lds, s, sb);
+    else
+        hipLaunchKernelGGL(rr_setup_batch_kernel<false>, dim3((unsigned)sh.setup_blocks), dim3(64), sh.setup_lds, s, sb);
+    rc = check_launch("rr_setup_batch_kernel");
     if (rc) return rc;
     const bool mixed = precision == RR_MIXED;
     for (int v = 0; v < 2; ++v) {
-        if (fb[v].n == 0) continue;
-        fb[v].start[fb[v].n] = (int)fblocks[v];
-        const dim3 grid((unsigned)fblocks[v]), block(kThreads);
+        if (bd.flux[v].n == 0) continue;
+        const dim3 grid((unsigned)sh.flux_blocks[v]), block(kThreads);
+        const size_t l = sh.flux_lds[v];
         if (mixed) {
-            if (v) hipLaunchKernelGGL((rr_flux_batch_kernel<true, true>), grid, block, lds[v], s, fb[v]);
-            else hipLaunchKernelGGL((rr_flux_batch_kernel<true, false>), grid, block, lds[v], s, fb[v]);
+            if (v) hipLaunchKernelGGL((rr_flux_batch_kernel<true, true>), grid, block, l, s, bd.flux[v]);
+            else hipLaunchKernelGGL((rr_flux_batch_kernel<true, false>), grid, block, l, s, bd.flux[v]);
         } else {
-            if (v) hipLaunchKernelGGL((rr_flux_batch_kernel<false, true>), grid, block, lds[v], s, fb[v]);
-            else hipLaunchKernelGGL((rr_flux_batch_kernel<false, false>), grid, block, lds[v], s, fb[v]);
+            if (v) hipLaunchKernelGGL((rr_flux_batch_kernel<false, true>), grid, block, l, s, bd.flux[v]);
+            else hipLaunchKernelGGL((rr_flux_batch_kernel<false, false>), grid, block, l, s, bd.flux[v]);
         }
         rc = check_launch("rr_flux_batch_kernel");
+        if (rc) return rc;
+    }
+    return RR_OK;
+}
+
+size_t rr_batch_desc_bytes(void) { return sizeof(BatchDesc); }
+
+int rr_batch_plan(int precision, const rr_table* tab, int nreq, const rr_request* reqs, void* desc,
+                  rr_batch_shape* shape) {
+    if (!desc || !shape) return set_error(RR_EINVAL, "rr_batch_plan: null descriptor or shape");
+    BatchDesc& bd = *(BatchDesc*)desc;
+    int rc = plan_batch("rr_batch_plan", precision, tab, nreq, reqs, bd, *shape);
+    if (rc) {
+        bd.setup.n = bd.flux[0].n = bd.flux[1].n = 0;
+        bd.setup.start[0] = bd.flux[0].start[0] = bd.flux[1].start[0] = 0;
+    }
+    return rc;
+}
+
+int rr_batch_launch(int precision, const void* desc_dev, const rr_batch_shape* launch, void* stream) {
+    if (!desc_dev || !launch) return set_error(RR_EINVAL, "rr_batch_launch: null descriptor or shape");
+    if (precision != RR_FP64 && precision != RR_MIXED) return set_error(RR_EINVAL, "rr_batch_launch: bad precision");
+    const rr_batch_shape& sh = *launch;
+    if (sh.setup_blocks < 0 || sh.flux_blocks[0] < 0 || sh.flux_blocks[1] < 0 || sh.setup_lds < 0 ||
+        sh.flux_lds[0] < 0 || sh.flux_lds[1] < 0 || sh.setup_lds > 65536 || sh.flux_lds[0] > 65536 ||
+        sh.flux_lds[1] > 65536)
+        return set_error(RR_EINVAL, "rr_batch_launch: bad launch shape");
+    const BatchDesc* bd = (const BatchDesc*)desc_dev;
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if (sh.setup_blocks > 0) {
+        if (sh.setup_multi)
+            hipLaunchKernelGGL(rr_setup_batch_dev_kernel<true>, dim3((unsigned)sh.setup_blocks), dim3(64),
+                               sh.setup_lds, s, &bd->setup);
+        else
+            hipLaunchKernelGGL(rr_setup_batch_dev_kernel<false>, dim3((unsigned)sh.setup_blocks), dim3(64),
+                               sh.setup_lds, s, &bd->setup);
+        rc = check_launch("rr_setup_batch_dev_kernel");
+        if (rc) return rc;
+    }
+    const bool mixed = precision == RR_MIXED;
+    for (int v = 0; v < 2; ++v) {
+        if (sh.flux_blocks[v] == 0) continue;
+        const dim3 grid((unsigned)sh.flux_blocks[v]), block(kThreads);
+        const size_t l = sh.flux_lds[v];
+        const FluxBatch* fb = &bd->flux[v];
+        if (mixed) {
+            if (v) hipLaunchKernelGGL((rr_flux_batch_dev_kernel<true, true>), grid, block, l, s, fb);
+            else hipLaunchKernelGGL((rr_flux_batch_dev_kernel<true, false>), grid, block, l, s, fb);
+        } else {
+            if (v) hipLaunchKernelGGL((rr_flux_batch_dev_kernel<false, true>), grid, block, l, s, fb);
+            else hipLaunchKernelGGL((rr_flux_batch_dev_kernel<false, false>), grid, block, l, s, fb);
+        }
+        rc = check_launch("rr_flux_batch_dev_kernel");
         if (rc) return rc;
     }
     return RR_OK;

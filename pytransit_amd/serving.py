@@ -20,6 +20,14 @@ This module adds one for many independent fits sharing a GPU:
   and at most two flux launches for all of them.  Requests that arrive while
   other batches run go into the next batch on another slot, so their work
   overlaps on the device.
+* **Replayed launches** (``graphs=True``).  Each slot keeps its batch
+  descriptor in a device buffer; a batch is planned on the host
+  (``rr_batch_plan``), its descriptor goes to the device with one asynchronous
+  copy, and the setup and flux launches (``rr_batch_launch``, which read the
+  descriptor from device memory) are a HIP graph captured once per slot and
+  launch shape and replayed for every later batch of that shape.  Shapes are
+  bucketed (block counts to the next power of two, dynamic LDS to 4 KB), blocks
+  past a batch's own exit at once, so few graphs serve a whole stream.
 * **Latency.**  Each request records its arrival, admission and completion times
   (host clock; completion is when the scheduler sees the batch's event).
 
@@ -34,7 +42,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import LD_LAW_IDS, RRLcData, RRLd, RRParams, RRRequest, check
+from ._lib import LD_LAW_IDS, RRBatchShape, RRLcData, RRLd, RRParams, RRRequest, check
 from .rrmodel import _PRECISIONS, RoadRunnerModel
 from .transitmodel import TransitModel, layout_flags
 
@@ -213,6 +221,8 @@ class RRServer:
     precision : 'mixed' or 'fp64', as RoadRunnerModel.
     on_done : called with each finished request; with ``auto_release`` its flux
         memory is returned right after the call.
+    graphs : replay each slot's setup and flux launches from HIP graphs captured
+        once per launch shape, with the batch descriptor in device memory.
     Table geometry keywords (klims, nk, nzin, nzlimb, zcut, ng) are
     RoadRunnerModel's.
     """
@@ -220,7 +230,8 @@ class RRServer:
     def __init__(self, hbm_budget: int = 16 << 30, slots: int = 4, max_batch: int = 16, precision: str = 'mixed',
                  device=None, on_done: Optional[Callable[[Request], None]] = None, auto_release: bool = False,
                  max_skips: int = 8, staging_bytes: int = 64 << 20, klims=(0.005, 0.5), nk=256, nzin=20,
-                 nzlimb=20, zcut=0.7, ng=100, executor=None, max_batch_points: Optional[int] = None):
+                 nzlimb=20, zcut=0.7, ng=100, executor=None, max_batch_points: Optional[int] = None,
+                 graphs: bool = False):
         self.precision = precision
         self.max_batch_points = max_batch_points
         self._prec = _PRECISIONS[precision]
@@ -235,7 +246,8 @@ class RRServer:
         self._exec = executor if executor is not None else _HipExecutor(self, hbm_budget, slots, max_batch,
                                                                          precision, device, staging_bytes,
                                                                          dict(klims=klims, nk=nk, nzin=nzin,
-                                                                              nzlimb=nzlimb, zcut=zcut, ng=ng))
+                                                                              nzlimb=nzlimb, zcut=zcut, ng=ng),
+                                                                         graphs)
         self.slots = slots
         self.max_batch = min(max_batch, self._exec.max_batch)
         self._free_slots = deque(range(slots))
@@ -406,7 +418,8 @@ class _HipExecutor:
     """Device side of the server: arena tensor, weight table, slot streams, pinned
     staging and the rr_evaluate_batch launches."""
 
-    def __init__(self, server, hbm_budget, slots, max_batch, precision, device, staging_bytes, table_kw):
+    def __init__(self, server, hbm_budget, slots, max_batch, precision, device, staging_bytes, table_kw,
+                 graphs=False):
         self.lib = _lib.load()
         self.server = server
         self.device = torch.device(device) if device is not None else torch.device('cuda', torch.cuda.current_device())
@@ -421,6 +434,14 @@ class _HipExecutor:
         self.staging = [torch.empty(staging_bytes, dtype=torch.uint8, pin_memory=True) for _ in range(slots)]
         self.upload_staging = None
         self.ng = self.model.ng
+        self.graphs = graphs
+        self.graph_captures = 0
+        self.graph_replays = 0
+        if graphs:
+            nb = int(self.lib.rr_batch_desc_bytes())
+            self.desc_host = [torch.empty(nb, dtype=torch.uint8, pin_memory=True) for _ in range(slots)]
+            self.desc_dev = [torch.empty(nb, dtype=torch.uint8, device=self.device) for _ in range(slots)]
+            self.slot_graphs = [{} for _ in range(slots)]   # bucketed shape -> graph, kept for the server's life
 
     def workspace_bytes(self, r: Request) -> int:
         d = r.dataset
@@ -486,11 +507,59 @@ class _HipExecutor:
             d.workspace = self.base + base + r.params.nbytes
             d.workspace_bytes = r.ws_bytes
             d.flux = self.base + r.flux_offset
-        check(lib.rr_evaluate_batch(self.server._prec, _lib.ctypes.byref(self.table), len(reqs), descs,
-                                    stream.cuda_stream))
+        if self.graphs:
+            self._replay(slot, stream, reqs, descs)
+        else:
+            check(lib.rr_evaluate_batch(self.server._prec, _lib.ctypes.byref(self.table), len(reqs), descs,
+                                        stream.cuda_stream))
         ev = torch.cuda.Event()
         ev.record(stream)
         return ev
+
+    @staticmethod
+    def _bucket_shape(sh):
+        """The launch shape a graph is captured for: block counts rounded up to a
+        power of two (at least 64), LDS bytes to a multiple of 4 KB."""
+        def blocks(n):
+            return 0 if n == 0 else max(64, 1 << (int(n) - 1).bit_length())
+
+        def lds(n):
+            return (int(n) + 4095) // 4096 * 4096
+        out = RRBatchShape(setup_blocks=blocks(sh.setup_blocks), setup_multi=sh.setup_multi,
+                           setup_lds=lds(sh.setup_lds))
+        for v in range(2):
+            out.flux_blocks[v] = blocks(sh.flux_blocks[v])
+            out.flux_lds[v] = lds(sh.flux_lds[v]) if sh.flux_blocks[v] else 0
+        key = (out.setup_blocks, out.setup_multi, out.setup_lds, out.flux_blocks[0], out.flux_lds[0],
+               out.flux_blocks[1], out.flux_lds[1])
+        return key, out
+
+    def _replay(self, slot, stream, reqs, descs):
+        """Plan the batch into the slot's pinned descriptor, copy it to the slot's
+        device descriptor, then replay (capturing first if new) the graph of the
+        bucketed launch shape.  The pinned descriptor is rewritten only once the
+        slot's previous batch has completed (the slot is free again only then)."""
+        lib = self.lib
+        prec = self.server._prec
+        shape = RRBatchShape()
+        check(lib.rr_batch_plan(prec, _lib.ctypes.byref(self.table), len(reqs), descs,
+                                self.desc_host[slot].data_ptr(), _lib.ctypes.byref(shape)))
+        with torch.cuda.stream(stream):
+            self.desc_dev[slot].copy_(self.desc_host[slot], non_blocking=True)
+        key, launch = self._bucket_shape(shape)
+        graphs = self.slot_graphs[slot]
+        g = graphs.get(key)
+        if g is None:
+            g = torch.cuda.CUDAGraph()
+            desc = self.desc_dev[slot].data_ptr()
+            with torch.cuda.graph(g, capture_error_mode='thread_local'):
+                check(lib.rr_batch_launch(prec, desc, _lib.ctypes.byref(launch),
+                                          torch.cuda.current_stream(self.device).cuda_stream))
+            graphs[key] = g
+            self.graph_captures += 1
+        with torch.cuda.stream(stream):
+            g.replay()
+        self.graph_replays += 1
 
     @staticmethod
     def finished(event) -> bool:

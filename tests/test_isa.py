@@ -53,7 +53,8 @@ def disassembly(tmp_path_factory):
 
 def test_no_packed_fp32(disassembly):
     text, nbundles = disassembly
-    assert nbundles == 4  # rr_kernels, ts_kernels, adjoint_kernels, ma_kernels
+    from pytransit_amd.build import SOURCES
+    assert nbundles == len(SOURCES)  # one offload bundle per HIP source
     bad = sorted(set(re.findall(r'\bv_pk_(?:fma|mul|add)_f32\b', text)))
     assert not bad, f'packed FP32 instructions in the device code: {bad}'
 
@@ -61,6 +62,6 @@ def test_no_packed_fp32(disassembly):
 def test_disassembly_covers_every_kernel_family(disassembly):
     text, _ = disassembly
     for kern in ('rr_flux_kernel', 'rr_setup_kernel', 'ts_ldm_mfma_kernel', 'ts_flux_rows_kernel',
-                 'rr_lnlike_kernel', 'ma_flux_kernel'):
+                 'rr_lnlike_kernel', 'ma_flux_kernel', 'ec_flux_kernel', 'op_flux_kernel'):
         assert kern in text, kern
     assert re.search(r'\bv_mfma_f32_16x16x4_?f32\b', text)

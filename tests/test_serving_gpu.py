@@ -52,6 +52,32 @@ def test_mix_equals_alone(gpu, prec):
                                                         f'npt={r.dataset.npt}, npb={r.dataset.npb})')
 
 
+@pytest.mark.parametrize('prec', ['mixed', 'fp64'])
+def test_graph_replay_equals_alone(gpu, prec):
+    # replayed launches (descriptor in device memory, graphs captured per bucketed
+    # launch shape): two passes over the same stream with eager evaluations and
+    # host synchronisations between them, so graphs captured in the first pass are
+    # replayed in the second; every request of both passes equals its eager
+    # evaluation alone, bit for bit
+    st = small_stream(seed=13)
+    say(f'case: serving with graph replay, {len(st.requests)} requests x 2 passes, {prec}')
+    dsets = [Dataset(**d) for d in st.datasets]
+    srv = RRServer(hbm_budget=1 << 30, slots=3, max_batch=16, precision=prec, graphs=True)
+    refs = [alone(st.datasets[rq['dataset']], rq, prec) for rq in st.requests]
+    for npass in range(2):
+        reqs = [submit(srv, dsets, rq) for rq in st.requests]
+        srv.run(timeout=60)
+        torch.cuda.synchronize()
+        for r, rq, ref in zip(reqs, st.requests, refs):
+            np.testing.assert_array_equal(r.result(), ref, err_msg=f'pass {npass} request {r.rid} ({rq["ld"]}, '
+                                                                   f'npv={r.npv}, npt={r.dataset.npt})')
+            r.release()
+        refs[0] = alone(st.datasets[st.requests[0]['dataset']], st.requests[0], prec)   # eager work between
+    ex = srv._exec
+    say(f'  graphs captured {ex.graph_captures}, replays {ex.graph_replays}')
+    assert ex.graph_replays > ex.graph_captures > 0
+
+
 def test_continuous_arrivals_tight_budget(gpu):
     st = small_stream(seed=12, n=30)
     say(f'case: serving, {len(st.requests)} requests in waves, budget for a few at a time')
