@@ -38,7 +38,7 @@ struct LnlArgs {
 };
 
 constexpr int kLnlThreads = 256;
-constexpr int kLnlMaxNg = 1024;
+constexpr int kLnlMaxNg = 512;
 constexpr int kLnlMaxNz = 1024;
 
 // Position at stencil time t and its derivative with respect to the period:
@@ -89,7 +89,8 @@ template <typename Real>
 __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
     __shared__ double s_x[7], s_y[7], s_dx[7], s_dy[7], s_c[10], s_dc[10], s_win[2];
     __shared__ double s_l[kLnlMaxNz], s_d0[kLnlMaxNz], s_d1[kLnlMaxNz];
-    __shared__ Real s_L[kLnlMaxNg], s_Lk[kLnlMaxNg], s_Lc0[kLnlMaxNg], s_Lc1[kLnlMaxNg];
+    __shared__ Real s_L[kLnlMaxNg], s_Lk[kLnlMaxNg], s_Lc0[kLnlMaxNg], s_Lc1[kLnlMaxNg], s_dL[kLnlMaxNg];
+    __shared__ double s_L64[kLnlMaxNg];
     __shared__ double s_red[kLnlThreads / 64][8];
     __shared__ double s_ist, s_dist0, s_dist1;
     __shared__ int s_ok;
@@ -146,11 +147,18 @@ __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
             b0 += u0 * s_d0[iz]; b1 += u1 * s_d0[iz];
             c0 += u0 * s_d1[iz]; c1 += u1 * s_d1[iz];
         }
-        s_L[ig] = (Real)((1.0 - ak) * a0 + ak * a1);
+        const double L = (1.0 - ak) * a0 + ak * a1;
+        s_L64[ig] = L;
+        s_L[ig] = (Real)L;
         s_Lk[ig] = (Real)((a1 - a0) / s.dk);
         s_Lc0[ig] = (Real)((1.0 - ak) * b0 + ak * b1);
         s_Lc1[ig] = (Real)((1.0 - ak) * c0 + ak * c1);
     }
+    __syncthreads();
+    // the g slope of each interval from the FP64 means: the difference of two
+    // rounded FP32 values would lose ~1e-5 of the slope, which the t0, p, a and i
+    // gradients carry through the chain rule
+    for (int ig = tid; ig < s.ng - 1; ig += kLnlThreads) s_dL[ig] = (Real)((s_L64[ig + 1] - s_L64[ig]) / s.dg);
     __syncthreads();
     if (!s_ok) {
         if (tid == 0) {
@@ -217,7 +225,7 @@ __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
                 i = i > s.ng - 2 ? s.ng - 2 : (i < 0 ? 0 : i);
                 const Real al = gg * inv_dg - (Real)i;
                 Ii = s_L[i] + al * (s_L[i + 1] - s_L[i]);
-                dIdg = (s_L[i + 1] - s_L[i]) * inv_dg;
+                dIdg = s_dL[i];
                 Ik = s_Lk[i] + al * (s_Lk[i + 1] - s_Lk[i]);
                 Ic0 = s_Lc0[i] + al * (s_Lc0[i + 1] - s_Lc0[i]);
                 Ic1 = s_Lc1[i] + al * (s_Lc1[i + 1] - s_Lc1[i]);

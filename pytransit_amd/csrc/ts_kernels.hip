@@ -12,8 +12,11 @@
 //   ts_setup_kernel    one 64-lane workgroup per pv (orbit on lanes 0-6, disk integrals)
 //   ts_weights_kernel  one lane per (pv, g): the 2-D weight row at k_mean, written
 //                      transposed W2T[pv][z][g] (FP64 and FP32, g padded to 16)
-//   ts_ldm_mfma_kernel FP32 MFMA contraction ldm = ldp . W2T, one workgroup per pv:
-//                      v_mfma_f32_16x16x4f32 tiles, W2T in LDS, profile rows streamed
+//   ts_ldm_tab_mfma_kernel  FP32 MFMA contraction ldm = ldp . W2T for tabulated FP64
+//                      profiles: one 8-wave workgroup per pv, W2T in LDS, each row
+//                      read as 128-byte pieces (16-byte loads, a permuted z order)
+//   ts_ldm_mfma_kernel the same for named laws (profiles evaluated per element) and
+//                      the shapes the first does not take
 //   ts_ldm_mfma64_kernel the FP64 contraction on v_mfma_f64_16x16x4f64 (W2T in LDS
 //                      when it fits in 128 KB), else ts_ldm_f64_kernel on the VALU
 //   ts_flux_kernel     one lane per time, a workgroup per (pv, 256 times): the FP64
@@ -257,6 +260,89 @@ __global__ void __launch_bounds__(256) ts_ldm_mfma_kernel(TsArgs s) {
                     const int pbo = rt * 16 + lk * 4 + rr;
                     if (pbo < s.npb && g < s.ng) s.ldm32[((size_t)pv * s.npb + pbo) * s.ng + g] = acc[c][rr];
                 }
+            }
+        }
+    }
+}
+
+// ---- FP32 MFMA contraction of tabulated FP64 profiles, row-contiguous loads ---------
+// ldm[pv][pb][g] = sum_z ldp[pv][pb][z] W2T[pv][z][g] for profiles given as an
+// FP64 table (the TSModel's LDModel input).  One workgroup of 8 waves per pv with
+// W2T in LDS; a wave takes one 16-passband row tile at a time.  The sum over z is
+// taken in a permuted order: lane (li, lk) holds row li at z = 16 j + 4 lk + q for
+// MFMA step 4 j + q, so each lane loads 4 consecutive FP64 values (two 16-byte
+// loads) and the four lanes of a row read 128 contiguous bytes; the B fragment
+// reads of LDS follow the same permutation.  The LDS row stride (4 mod 8 floats)
+// puts the four z rows of an MFMA step in distinct banks.
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+constexpr int kTabMfmaThreads = 512;
+constexpr int kTabMfmaMaxJ = 8;    // z groups of 16: nz <= 128
+constexpr int kTabMfmaMaxCt = 8;   // g tiles of 16: ngp <= 128
+
+__host__ __device__ inline int ts_tab_ldb(int ngp) { return ngp + ((4 - ngp % 8) + 8) % 8; }
+
+template <int NCT>
+__global__ void __launch_bounds__(kTabMfmaThreads) ts_ldm_tab_mfma_kernel(TsArgs s) {
+    extern __shared__ float lds_t[];  // [nz16][ldb]
+    const int pv = blockIdx.x;
+    const int ldb = ts_tab_ldb(s.ngp);
+    const int nj = (s.nz + 15) / 16, nz16 = nj * 16;
+    for (int i = threadIdx.x; i < nz16 * s.ngp; i += blockDim.x) {
+        const int iz = i / s.ngp, ig = i - iz * s.ngp;
+        lds_t[iz * ldb + ig] = iz < s.nzp ? s.w2_32[(size_t)pv * s.nzp * s.ngp + i] : 0.0f;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int nrt = (s.npb + 15) / 16;
+    constexpr int kWaves = kTabMfmaThreads / 64;
+    for (int rt = wave; rt < nrt; rt += kWaves) {
+        const int pb = rt * 16 + li;
+        const bool row_ok = pb < s.npb;
+        const double* arow = s.ldp_in + ((size_t)pv * s.npb + (row_ok ? pb : 0)) * s.nz;
+        f32x4_t acc[NCT];
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) acc[c] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+        // the row in two halves of 64 z values: 16 FP64 values in flight per lane
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            constexpr int kJh = kTabMfmaMaxJ / 2;
+            float af[4 * kJh];
+#pragma unroll
+            for (int jj = 0; jj < kJh; ++jj) {
+                const int j = h * kJh + jj, z0 = 16 * j + 4 * lk;
+                f64x2_t v0 = {0.0, 0.0}, v1 = {0.0, 0.0};
+                if (j < nj && row_ok && z0 < s.nz) {
+                    v0 = *(const f64x2_t*)(arow + z0);
+                    v1 = *(const f64x2_t*)(arow + z0 + 2);
+                }
+                af[4 * jj + 0] = (float)v0.x;
+                af[4 * jj + 1] = (float)v0.y;
+                af[4 * jj + 2] = (float)v1.x;
+                af[4 * jj + 3] = (float)v1.y;
+            }
+#pragma unroll
+            for (int jj = 0; jj < kJh; ++jj) {
+                const int j = h * kJh + jj;
+                if (j < nj) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float* brow = lds_t + (16 * j + 4 * lk + q) * ldb + li;
+#pragma unroll
+                        for (int c = 0; c < NCT; ++c)
+                            acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[4 * jj + q], brow[c * 16], acc[c], 0, 0,
+                                                                          0);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) {
+            const int g = c * 16 + li;
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const int pbo = rt * 16 + lk * 4 + rr;
+                if (pbo < s.npb && g < s.ng) s.ldm32[((size_t)pv * s.npb + pbo) * s.ng + g] = acc[c][rr];
             }
         }
     }
@@ -717,7 +803,26 @@ int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
     const long long nw = (long long)npv * a.ngp;
     hipLaunchKernelGGL(ts_weights_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, a);
     if ((rc = check_launch("ts_weights_kernel"))) return rc;
-    if (mixed) {
+    // tabulated FP64 profiles with nz a multiple of 4 (16-byte aligned rows), nz <= 128
+    // and ngp <= 128: the row-contiguous kernel
+    const int nct = a.ngp / 16;
+    const bool tab_rows = mixed && tab_ld && a.nz % 4 == 0 && a.nz <= 16 * kTabMfmaMaxJ && nct <= kTabMfmaMaxCt &&
+                          ((uintptr_t)ld->ldp & 15) == 0;
+    if (tab_rows) {
+        const size_t lds = (size_t)((a.nz + 15) / 16 * 16) * ts_tab_ldb(a.ngp) * sizeof(float);
+        const dim3 g(npv), b(kTabMfmaThreads);
+        switch (nct) {
+            case 1: hipLaunchKernelGGL(ts_ldm_tab_mfma_kernel<1>, g, b, lds, st, a); break;
+            case 2: hipLaunchKernelGGL(ts_ldm_tab_mfma_kernel<2>, g, b, lds, st, a); break;
+            case 3: hipLaunchKernelGGL(ts_ldm_tab_mfma_kernel<3>, g, b, lds, st, a); break;
+            case 4: hipLaunchKernelGGL(ts_ldm_tab_mfma_kernel<4>, g, b, lds, st, a); break;
+            case 5: hipLaunchKernelGGL(ts_ldm_tab_mfma_kernel<5>, g, b, lds, st, a); break;
+            case 6: hipLaunchKernelGGL(ts_ldm_tab_mfma_kernel<6>, g, b, lds, st, a); break;
+            case 7: hipLaunchKernelGGL(ts_ldm_tab_mfma_kernel<7>, g, b, lds, st, a); break;
+            default: hipLaunchKernelGGL(ts_ldm_tab_mfma_kernel<8>, g, b, lds, st, a); break;
+        }
+        if ((rc = check_launch("ts_ldm_tab_mfma_kernel"))) return rc;
+    } else if (mixed) {
         const size_t lds = (size_t)a.nzp * (a.ngp + 1) * sizeof(float);
         if (lds > 160 * 1024) return set_error(RR_ERANGE, "ts_evaluate: nz*ng too large for the MFMA kernel");
         if (tab_ld) hipLaunchKernelGGL(ts_ldm_mfma_kernel<true>, dim3(npv), dim3(256), lds, st, a);
