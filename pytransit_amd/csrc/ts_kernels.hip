@@ -496,10 +496,22 @@ __device__ __forceinline__ void ts_store(Real* o, const Real* v, int n, bool vec
 // npt <= 8192 (the usual transmission-spectroscopy shape: a few thousand times,
 // hundreds of passbands), one 1024-thread workgroup per pv writes the pv's
 // [npb][npt] block front to back: thread i owns the 16-byte time vectors i and
-// i + 1024 of every row, keeps their geometry in registers, and each step writes
-// one whole row.  The arithmetic per value is that of the tile form's cached path.
+// i + 1024 of every row, and each step writes one whole row.
+//
+// A row costs few operations per value: the geometry of each time is reduced once
+// to the LDS offset of its limb-darkening interval and five coefficients, with
+//   I = rb l[ig] + ra l[ig+1],  area = c1 afac + (c2 + c3 (k - k_mean)),
+//   F = 1 - (I area) / I*
+// (c1 = A0 inside z <= 1 - k_max, else c2 = A0 and c3 = 2 k_mean kappa0, the
+// serial model's two branches); outside the window, and for g > 1 where both the
+// mean and the area vanish, F = 1 is selected, so a NaN row of limb-darkening
+// means reaches only the in-window values, as in the reference.  The LD rows of a chunk of passbands sit in LDS transposed,
+// passband fastest, so the rows of a 16-passband group are read with immediate
+// offsets from one address.  A wave none of whose times is in the window stores
+// ones without arithmetic.
 constexpr int kTsRowThreads = 1024;
 constexpr int kTsRowPc = 64;      // passbands per staged chunk
+constexpr int kTsRowQ = 16;       // passbands per unrolled group
 // time vectors (16 bytes: 4 FP32 or 2 FP64 times) per thread; npt <= 8192 (FP32), 6144 (FP64)
 template <typename Real> constexpr int ts_row_vec() { return sizeof(Real) == 4 ? 2 : 3; }
 template <typename Real> constexpr int ts_row_max_npt() { return kTsRowThreads * ts_row_vec<Real>() * ts_tpl<Real>(); }
@@ -511,7 +523,7 @@ __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_kernel(TsArgs s, R
     constexpr int TPL = ts_tpl<Real>();
     constexpr int NV = ts_row_vec<Real>();
     extern __shared__ __align__(16) unsigned char lds[];
-    Real* s_ldm = (Real*)lds;                        // [kTsRowPc][ng]
+    Real* s_ldm = (Real*)lds;                        // [ng][kTsRowPc]
     Real* s_afac = s_ldm + (size_t)kTsRowPc * s.ng;
     Real* s_kd = s_afac + kTsRowPc;
     Real* s_iist = s_kd + kTsRowPc;
@@ -520,74 +532,103 @@ __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_kernel(TsArgs s, R
     const int npt = s.npt, nv = npt / TPL;
     const TsRec& r = s.rec[pv];
     Real* out = flux + (size_t)pv * s.npb * npt;
-    Real vnan[TPL];
+    Real vnan[TPL], vone[TPL];
 #pragma unroll
-    for (int j = 0; j < TPL; ++j) vnan[j] = Real(NAN);
+    for (int j = 0; j < TPL; ++j) {
+        vnan[j] = Real(NAN);
+        vone[j] = Real(1);
+    }
     if (!r.good) {
         for (size_t i = threadIdx.x; i < (size_t)s.npb * nv; i += kTsRowThreads) ts_store(out + i * TPL, vnan, TPL, true);
         return;
     }
-    // the geometry of the thread's times, once
+    // the geometry of the thread's times, once, as row coefficients
     const double ext = s.exptimes[0];
-    const double km = r.kmean;
-    TsGeo<Real> geo[NV][TPL];
-    bool inw[NV][TPL];
+    int off[NV][TPL];
+    Real rb[NV][TPL], ra[NV][TPL], c1[NV][TPL], c2[NV][TPL], c3[NV][TPL];
+    bool use[NV][TPL];   // in the window with g <= 1
+    bool act[NV];        // the wave has a time of vector u in the window
 #pragma unroll
     for (int u = 0; u < NV; ++u) {
         const int v = threadIdx.x + u * kTsRowThreads;
+        bool any = false;
 #pragma unroll
         for (int j = 0; j < TPL; ++j) {
-            inw[u][j] = false;
-            geo[u][j].ra = geo[u][j].rb = geo[u][j].ap0 = geo[u][j].dadk = Real(0);
-            geo[u][j].ig = 0;
-            geo[u][j].inside = geo[u][j].gz = false;
+            off[u][j] = 0;
+            rb[u][j] = ra[u][j] = c1[u][j] = c2[u][j] = c3[u][j] = Real(0);
+            use[u][j] = false;
             if (v < nv) {
+                double tc;
+                {
 #pragma clang fp contract(off)
-                const double t = s.times[TPL * v + j];
-                const double t0v = s.t0[pv];
-                const double epoch = floor((t - t0v + 0.5 * r.p) * r.inv_p);
-                const double uu = t - t0v + 0.5 * r.p;
-                const double rem = uu - epoch * r.p;
-                const double ep = epoch + ((rem < 0.0) ? -1.0 : ((rem >= r.p) ? 1.0 : 0.0));
-                const double tc = t - (t0v + ep * r.p);
-                inw[u][j] = (r.bt1 <= tc) && (tc <= r.bt4);
-                if (inw[u][j]) geo[u][j] = ts_geometry<Real>(r, tc, ext, 1, 0, s.dg, s.ng);
+                    const double t = s.times[TPL * v + j];
+                    const double t0v = s.t0[pv];
+                    const double epoch = floor((t - t0v + 0.5 * r.p) * r.inv_p);
+                    const double uu = t - t0v + 0.5 * r.p;
+                    const double rem = uu - epoch * r.p;
+                    const double ep = epoch + ((rem < 0.0) ? -1.0 : ((rem >= r.p) ? 1.0 : 0.0));
+                    tc = t - (t0v + ep * r.p);
+                }
+                if ((r.bt1 <= tc) && (tc <= r.bt4)) {
+                    const TsGeo<Real> g = ts_geometry<Real>(r, tc, ext, 1, 0, s.dg, s.ng);
+                    any = true;
+                    if (!g.gz) {
+                        use[u][j] = true;
+                        off[u][j] = g.ig * kTsRowPc;
+                        rb[u][j] = g.rb;
+                        ra[u][j] = g.ra;
+                        c1[u][j] = g.inside ? g.ap0 : Real(0);
+                        c2[u][j] = g.inside ? Real(0) : g.ap0;
+                        c3[u][j] = g.inside ? Real(0) : g.dadk;
+                    }
+                }
             }
         }
+        act[u] = __ballot(any) != 0ull;
     }
     for (int pb0 = 0; pb0 < s.npb; pb0 += kTsRowPc) {
         const int npc = min(kTsRowPc, s.npb - pb0);
         __syncthreads();  // the previous chunk's rows are no longer read
-        for (int i = threadIdx.x; i < npc * s.ng; i += kTsRowThreads) {
-            const size_t src = ((size_t)pv * s.npb + pb0) * s.ng + i;
-            s_ldm[i] = sizeof(Real) == 4 ? (Real)s.ldm32[src] : (Real)s.ldm64[src];
+        for (int i = threadIdx.x; i < kTsRowPc * s.ng; i += kTsRowThreads) {
+            const int q = i / s.ng, g = i - q * s.ng;
+            const size_t src = ((size_t)pv * s.npb + pb0 + q) * s.ng + g;
+            s_ldm[g * kTsRowPc + q] = q < npc ? (sizeof(Real) == 4 ? (Real)s.ldm32[src] : (Real)s.ldm64[src]) : Real(0);
         }
         for (int q = threadIdx.x; q < npc; q += kTsRowThreads) {
             const double kq = s.k[(size_t)pv * s.npb + pb0 + q];
-            s_afac[q] = (Real)(kq * kq / (km * km));
-            s_kd[q] = (Real)(kq - km);
+            s_afac[q] = (Real)(kq * kq / (r.kmean * r.kmean));
+            s_kd[q] = (Real)(kq - r.kmean);
             s_iist[q] = (Real)(1.0 / s.istar[(size_t)pv * s.npb + pb0 + q]);
         }
         __syncthreads();
 #pragma unroll 1
-        for (int q = 0; q < npc; ++q) {
-            const Real* l = s_ldm + (size_t)q * s.ng;
-            const Real f = s_iist[q] * Real(1);  // one sample: inv_ns = 1
-            const Real afac = s_afac[q], kd = s_kd[q];
-            Real* o = out + (size_t)(pb0 + q) * npt;
+        for (int q0 = 0; q0 < npc; q0 += kTsRowQ) {
+            const Real* lq = s_ldm + q0;
 #pragma unroll
-            for (int u = 0; u < NV; ++u) {
-                const int v = threadIdx.x + u * kTsRowThreads;
-                if (v < nv) {
-                    Real w[TPL];
+            for (int qq = 0; qq < kTsRowQ; ++qq) {
+                const int q = q0 + qq;
+                if (q < npc) {
+                    const Real f = s_iist[q], afac = s_afac[q], kd = s_kd[q];
+                    Real* o = out + (size_t)(pb0 + q) * npt;
 #pragma unroll
-                    for (int j = 0; j < TPL; ++j) {
-                        const TsGeo<Real>& g = geo[u][j];
-                        const Real ip = g.gz ? Real(0) : (g.rb * l[g.ig] + g.ra * l[g.ig + 1]);
-                        const Real area = g.inside ? g.ap0 * afac : g.ap0 + kd * g.dadk;
-                        w[j] = inw[u][j] ? Real(1) - (ip * area) * f : Real(1);
+                    for (int u = 0; u < NV; ++u) {
+                        const int v = threadIdx.x + u * kTsRowThreads;
+                        if (v < nv) {
+                            if (act[u]) {
+                                Real w[TPL];
+#pragma unroll
+                                for (int j = 0; j < TPL; ++j) {
+                                    const Real* l = lq + off[u][j] + qq;
+                                    const Real ip = rb[u][j] * l[0] + ra[u][j] * l[kTsRowPc];
+                                    const Real area = c1[u][j] * afac + (c2[u][j] + c3[u][j] * kd);
+                                    w[j] = use[u][j] ? Real(1) - (ip * area) * f : Real(1);
+                                }
+                                ts_store(o + (size_t)v * TPL, w, TPL, true);
+                            } else {
+                                ts_store(o + (size_t)v * TPL, vone, TPL, true);
+                            }
+                        }
                     }
-                    ts_store(o + (size_t)v * TPL, w, TPL, true);
                 }
             }
         }
