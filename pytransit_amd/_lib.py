@@ -152,6 +152,20 @@ def check(rc: int):
         raise RRError(f'librr_hip error {rc}: {msg}')
 
 
+def to_host(t: torch.Tensor):
+    """A device tensor copied to a new numpy array through page-locked memory.
+
+    The copy goes at the link's speed (a pageable destination takes the driver's
+    bounce-buffer path, several times slower for flux arrays of hundreds of MB);
+    torch's caching host allocator hands the page-locked block back for reuse
+    once the returned array is released."""
+    if t.device.type != 'cuda':
+        return t.numpy()
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t)
+    return h.numpy()
+
+
 def ptr(t) -> int:
     """Device (or host) address of a tensor, 0 for None."""
     return 0 if t is None else t.data_ptr()

@@ -77,7 +77,7 @@ class EclipseModel(_EclipseBase):
         parameter vectors (a <= 1, a NaN, e < 0) give NaN rows.
         """
         flux = self._evaluate_device(k, t0, p, a, i, e, w, rstar).squeeze()
-        return flux.cpu().numpy() if copy else flux
+        return _lib.to_host(flux) if copy else flux
 
     def __call__(self, *args, **kwargs):
         return self.evaluate(*args, **kwargs)
@@ -126,7 +126,7 @@ class EclipseSpectroscopyModel(_EclipseBase):
         ``copy=False`` the device tensor is returned.
         """
         flux = self._evaluate_device(f, k, t0, p, a, i, e, w, rstar)
-        return flux.cpu().numpy() if copy else flux
+        return _lib.to_host(flux) if copy else flux
 
     def __call__(self, f, k, t0, p, a, i, e=0.0, w=0.0, rstar=1.0):
         return self.evaluate(f, k, t0, p, a, i, e, w, rstar)

@@ -47,7 +47,7 @@ class OblatePlanetModel(RoadRunnerModel):
         parameter vectors give NaN rows.
         """
         flux = self._evaluate_op(k, f, alpha, ldc, t0, p, a, i, e, w).squeeze()
-        return flux.cpu().numpy() if copy else flux
+        return _lib.to_host(flux) if copy else flux
 
     def __call__(self, *args, **kwargs):
         return self.evaluate(*args, **kwargs)

@@ -50,7 +50,7 @@ class QuadraticModel(TransitModel):
         tensor otherwise).  A NaN radius ratio or semi-major axis gives an inf row.
         """
         flux = self._evaluate_device(k, ldc, t0, p, a, i, e, w).squeeze()
-        return flux.cpu().numpy() if copy else flux
+        return _lib.to_host(flux) if copy else flux
 
     def _evaluate_device(self, k, ldc, t0, p, a, i, e=None, w=None):
         lib = _lib.load()

@@ -304,7 +304,7 @@ class RoadRunnerModel(TransitModel):
         limb darkening) give NaN rows (model_full.py:40-42,80-82).
         """
         flux = self._evaluate_device(k, ldc, t0, p, a, i, e, w).squeeze()
-        return flux.cpu().numpy() if copy else flux
+        return _lib.to_host(flux) if copy else flux
 
     def _descriptors(self, k, ldc, t0, p, a, i, e, w):
         """The C-ABI descriptors of one evaluate call: (npv, RRTable, RRLcData,

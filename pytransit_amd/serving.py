@@ -26,8 +26,9 @@ This module adds one for many independent fits sharing a GPU:
   copy, and the setup and flux launches (``rr_batch_launch``, which read the
   descriptor from device memory) are a HIP graph captured once per slot and
   launch shape and replayed for every later batch of that shape.  Shapes are
-  bucketed (block counts to the next power of two, dynamic LDS to 4 KB), blocks
-  past a batch's own exit at once, so few graphs serve a whole stream.
+  bucketed (block counts to the next power of two of at least 1024, dynamic LDS
+  to 8 KB), blocks past a batch's own exit at once, so few graphs serve a whole
+  stream.
 * **Latency.**  Each request records its arrival, admission and completion times
   (host clock; completion is when the scheduler sees the batch's event).
 
@@ -192,7 +193,7 @@ class Request:
     def result(self, copy: bool = True):
         if not self.done:
             raise RuntimeError('request not finished')
-        return self.flux.cpu().numpy() if copy else self.flux
+        return _lib.to_host(self.flux) if copy else self.flux
 
     def release(self) -> None:
         if self._server is not None and self.flux_offset is not None:
@@ -442,6 +443,7 @@ class _HipExecutor:
             self.desc_host = [torch.empty(nb, dtype=torch.uint8, pin_memory=True) for _ in range(slots)]
             self.desc_dev = [torch.empty(nb, dtype=torch.uint8, device=self.device) for _ in range(slots)]
             self.slot_graphs = [{} for _ in range(slots)]   # bucketed shape -> graph, kept for the server's life
+            self.capture_stream = torch.cuda.Stream(self.device)
 
     def workspace_bytes(self, r: Request) -> int:
         d = r.dataset
@@ -519,12 +521,14 @@ class _HipExecutor:
     @staticmethod
     def _bucket_shape(sh):
         """The launch shape a graph is captured for: block counts rounded up to a
-        power of two (at least 64), LDS bytes to a multiple of 4 KB."""
+        power of two (at least 1024: blocks past a batch's own exit at once, so a
+        small batch costs a few microseconds more on a larger grid), LDS bytes to a
+        multiple of 8 KB.  Coarse buckets keep the graphs of a stream few."""
         def blocks(n):
-            return 0 if n == 0 else max(64, 1 << (int(n) - 1).bit_length())
+            return 0 if n == 0 else max(1024, 1 << (int(n) - 1).bit_length())
 
         def lds(n):
-            return (int(n) + 4095) // 4096 * 4096
+            return (int(n) + 8191) // 8192 * 8192
         out = RRBatchShape(setup_blocks=blocks(sh.setup_blocks), setup_multi=sh.setup_multi,
                            setup_lds=lds(sh.setup_lds))
         for v in range(2):
@@ -550,11 +554,18 @@ class _HipExecutor:
         graphs = self.slot_graphs[slot]
         g = graphs.get(key)
         if g is None:
+            # captured on a stream of its own that runs nothing else, without the
+            # device synchronisation and garbage collection of torch.cuda.graph():
+            # the batches in flight on other slots are not waited for
             g = torch.cuda.CUDAGraph()
             desc = self.desc_dev[slot].data_ptr()
-            with torch.cuda.graph(g, capture_error_mode='thread_local'):
-                check(lib.rr_batch_launch(prec, desc, _lib.ctypes.byref(launch),
-                                          torch.cuda.current_stream(self.device).cuda_stream))
+            with torch.cuda.stream(self.capture_stream):
+                g.capture_begin(capture_error_mode='thread_local')
+                try:
+                    check(lib.rr_batch_launch(prec, desc, _lib.ctypes.byref(launch),
+                                              self.capture_stream.cuda_stream))
+                finally:
+                    g.capture_end()
             graphs[key] = g
             self.graph_captures += 1
         with torch.cuda.stream(stream):

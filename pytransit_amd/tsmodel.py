@@ -38,7 +38,7 @@ class TransmissionSpectroscopyModel(RoadRunnerModel):
         give NaN (model_trspec.py:37-39).
         """
         flux = self._evaluate_device(k, ldc, t0, p, a, i, e, w)
-        return flux.cpu().numpy() if copy else flux
+        return _lib.to_host(flux) if copy else flux
 
     def _evaluate_device(self, k, ldc, t0, p, a, i, e=0.0, w=0.0, flux_events=None):
         lib = _lib.load()
