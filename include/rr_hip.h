@@ -6,8 +6,9 @@
  * hipStream_t passed as `void*` (NULL = the default stream).  No exceptions cross
  * the ABI: each call returns RR_OK (0) or a negative error code, and
  * rr_last_error() returns the text of the calling thread's last error.  The
- * library keeps no state between calls apart from what the caller passes in, so
- * calls on different streams are independent.
+ * library keeps no state between calls apart from what the caller passes in (and
+ * the per-GPU scratch array of rr_evaluate_noalloc), so calls on different
+ * streams are independent (rr_evaluate_noalloc's excepted).
  *
  * Reference citations are to PyTransit 2.8.1 (paths relative to its checkout).
  */
@@ -291,6 +292,18 @@ int es_evaluate(const rr_lcdata* lc, const rr_params* pv, const double* rstar, c
 int op_evaluate(const rr_table* tab, const rr_lcdata* lc, const rr_params* pv, const rr_ld* ld, const double* f,
                 const double* alpha, int nlines, void* workspace, size_t workspace_bytes, double* flux,
                 void* stream);
+
+/* rr_evaluate without allocation: the workspace is a device array inside the
+ * library, RR_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; the
+ * caller passes inputs and the flux buffer only, and no memory is allocated or
+ * freed.  Requests whose workspace (rr_workspace_bytes) exceeds the array fail
+ * with RR_ERANGE.  Calls that overlap on one GPU share the array and race:
+ * serialise them (one stream, or events).  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). */
+#define RR_NOALLOC_SCRATCH_BYTES (32u << 20)
+int rr_evaluate_noalloc(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
+                        const rr_ld* ld, void* flux, void* stream);
 
 #ifdef __cplusplus
 }

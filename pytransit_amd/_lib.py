@@ -103,7 +103,10 @@ _SIGNATURES = {
                             c_void_p, c_void_p]),
     'op_evaluate': (c_int, [POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd), c_void_p,
                             c_void_p, c_int, c_void_p, c_size_t, c_void_p, c_void_p]),
+    'rr_evaluate_noalloc': (c_int, [c_int, POINTER(RRTable), POINTER(RRLcData), POINTER(RRParams), POINTER(RRLd),
+                                    c_void_p, c_void_p]),
 }
+RR_NOALLOC_SCRATCH_BYTES = 32 << 20
 
 _lib = None
 

@@ -82,6 +82,18 @@ int main() {
     CHECK(std::strstr(rr_last_error(), "[7]") != nullptr);
     CHECK(rr_batch_launch(RR_MIXED, desc.data(), nullptr, nullptr) == RR_EINVAL);
 
+    // the allocation-free entry point refuses problems above its scratch array
+    rr_lcdata lc;
+    std::memset(&lc, 0, sizeof(lc));
+    lc.npt = 1000; lc.nlc = 1; lc.npb = 1; lc.nep = 1;
+    rr_params pv;
+    std::memset(&pv, 0, sizeof(pv));
+    pv.npv = 50000; pv.kcols = 1; pv.t0cols = 1;
+    rr_ld ld;
+    std::memset(&ld, 0, sizeof(ld));
+    ld.law = RR_LD_QUADRATIC; ld.nldc = 2;
+    CHECK(rr_evaluate_noalloc(RR_FP64, &tab, &lc, &pv, &ld, nullptr, nullptr) == RR_ERANGE);
+
     if (failures) {
         std::fprintf(stderr, "%d check(s) failed\n", failures);
         return 1;

@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -2043,6 +2044,59 @@ int rr_batch_launch(int precision, const void* desc_dev, const rr_batch_shape* l
         if (rc) return rc;
     }
     return RR_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// Evaluation without allocation.  The per-pv workspace (records, disk integrals,
+// limb-darkening means) lives in a file-scope device array of RR_NOALLOC_SCRATCH_BYTES.
+// Every GPU holds its own copy at its own address, found once per device with
+// hipGetSymbolAddress; host code uses only that address, never the array's name
+// (which on the host is a stand-in, not a device address).  The setup kernel
+// writes every workspace byte the flux kernel reads, so nothing needs clearing
+// between calls; calls that overlap on one GPU share the array and race.
+static __device__ __align__(256) unsigned char scratch[RR_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(void** out) {
+    int dev = -1;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return set_error(RR_EHIP, "rr_evaluate_noalloc: hipGetDevice: %s", hipGetErrorString(err));
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(RR_ERANGE, "rr_evaluate_noalloc: device %d beyond the %d supported", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !p)
+            return set_error(RR_EHIP, "rr_evaluate_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                             hipGetErrorString(err));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return RR_OK;
+}
+}  // namespace
+
+int rr_evaluate_noalloc(int precision, const rr_table* tab, const rr_lcdata* lc, const rr_params* pv,
+                        const rr_ld* ld, void* flux, void* stream) {
+    if (!tab || !lc || !pv) return set_error(RR_EINVAL, "rr_evaluate_noalloc: null descriptor");
+    if (pv->npv < 0 || lc->npt < 0 || lc->npb < 1 || tab->ng < 0)
+        return set_error(RR_EINVAL, "rr_evaluate_noalloc: bad sizes (npv=%d npt=%d npb=%d ng=%d)", pv->npv, lc->npt,
+                         lc->npb, tab->ng);
+    const size_t need = rr_workspace_bytes(pv->npv, lc->npt, lc->npb, tab->ng, lc->nlc);
+    if (need > (size_t)RR_NOALLOC_SCRATCH_BYTES)
+        return set_error(RR_ERANGE,
+                         "rr_evaluate_noalloc: npv=%d x npb=%d x ng=%d needs %zu bytes of workspace, above the "
+                         "%zu-byte scratch array; use rr_evaluate with a caller-allocated workspace",
+                         pv->npv, lc->npb, tab->ng, need, (size_t)RR_NOALLOC_SCRATCH_BYTES);
+    void* ws = nullptr;
+    int rc = scratch_address(&ws);
+    if (rc) return rc;
+    return rr_evaluate(precision, tab, lc, pv, ld, ws, (size_t)RR_NOALLOC_SCRATCH_BYTES, flux, stream);
 }
 
 }  // extern "C"
