@@ -85,6 +85,37 @@ __device__ __forceinline__ Real dhorner5(const Real* c, Real t) {
     return ((Real(4) * c[4] * t + Real(3) * c[3]) * t + Real(2) * c[2]) * t + c[1];
 }
 
+// Lens area of the unit disk and a disk of radius k < 1 at separation z, in the
+// lens case |z - 1| < k, from u = z - 1 given to full FP32 precision.  With the
+// kite area written as akite = z sin kappa1 (Heron), the kite form
+// A = kappa1 + k^2 kappa0 - akite becomes
+//   A = k^2 kappa0 + S(kappa1) - u akite / z,   S(x) = x - sin x,
+// so a small planet's nearly equal kappa1 and akite are not subtracted (in FP32
+// that cancellation leaves ~1e-7 of the area's scale, 0.1 k); S is its Taylor
+// series up to x^11 for kappa1 <= 0.6 (truncation < 1e-13), the kite form above
+// it.  The kite's square root is ((1 + z)^2 - k^2)(k^2 - u^2), the sorted-sides
+// product of common.py:5-33,52-73 regrouped, and z^2 - 1 = u (2 + u).  Returns A,
+// kappa0 and akite.
+__device__ __forceinline__ void lens_seg_f32(float k, float z, float u, float& area, float& kap0, float& akite) {
+    const float zp1 = 1.0f + z;
+    const float sq = __builtin_amdgcn_sqrtf((zp1 * zp1 - k * k) * ((k - u) * (k + u)));  // 2 akite
+    const float zm = u * (2.0f + u);                                                     // z^2 - 1
+    kap0 = atan2_ypos_f(sq, zm + k * k);
+    const float kap1 = atan2_ypos_f(sq, 2.0f + zm - k * k);
+    akite = 0.5f * sq;
+    if (kap1 <= 0.6f) {
+        const float x2 = kap1 * kap1;
+        float sr = 1.0f / 39916800.0f;
+        sr = fmaf(sr, x2, -1.0f / 362880.0f);
+        sr = fmaf(sr, x2, 1.0f / 5040.0f);
+        sr = fmaf(sr, x2, -1.0f / 120.0f);
+        sr = fmaf(sr, x2, 1.0f / 6.0f);
+        area = k * k * kap0 + kap1 * x2 * sr - u * akite / z;
+    } else {
+        area = kap1 + k * k * kap0 - akite;
+    }
+}
+
 template <typename Real>
 __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
     __shared__ double s_x[7], s_y[7], s_dx[7], s_dy[7], s_c[10], s_dc[10], s_win[2];
@@ -95,6 +126,7 @@ __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
     __shared__ double s_ist, s_dist0, s_dist1;
     __shared__ int s_ok;
 
+    constexpr bool kMixed = std::is_same<Real, float>::value;
     const int pv = blockIdx.x;
     const int tid = threadIdx.x;
     const double a = s.a[pv], e = s.e[pv], p = s.p[pv], inc = s.inc[pv], w = s.w[pv];
@@ -185,6 +217,7 @@ __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
     }
     const Real kr = (Real)kk, zlim = Real(1) + kr, inv1pk = Real(1) / zlim;
     const Real dg = (Real)s.dg, inv_dg = (Real)(1.0 / s.dg);
+    const double g_scale = 1.0 / ((1.0 + kk) * s.dg);   // g / dg = z g_scale
     const Real tan_i = (Real)tan(inc), inv_a = (Real)(1.0 / a);
     const Real pik2 = (Real)(kPi * kk * kk), dpik2 = (Real)(2.0 * kPi * kk);
     const double scale = 1.0 / (ns * s_ist);
@@ -211,19 +244,49 @@ __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
         Real D = 0, dDk = 0, dDt0 = 0, dDp = 0, dDa = 0, dDi = 0, dDc0 = 0, dDc1 = 0;
         const Real ep = (Real)epoch;
         for (int is = 1; is <= ns; ++is) {
-            const Real tau = (Real)(tc + ext * ((is - 0.5) / ns - 0.5));
-            const Real x = horner5(cx, tau), y = horner5(cy, tau);
-            const Real z = sqrt(x * x + y * y);
+            const double taud = tc + ext * ((is - 0.5) / ns - 0.5);
+            const Real tau = (Real)taud;
+            Real x, y, z, u;
+            if constexpr (kMixed) {
+                // the constant and linear terms of the position in FP64: the phase
+                // resolves ~1e-9 d that an FP32 time would drop, ~1e-7 of the
+                // position at sky speeds of ~40 stellar radii per day.  z - 1 is
+                // formed from the FP64 squared distance, so the lens area below
+                // does not see the rounding of z.
+                const Real t2 = tau * tau;
+                const double xd = s_c[0] + s_c[1] * taud + (double)(((cx[4] * tau + cx[3]) * tau + cx[2]) * t2);
+                const double yd = s_c[5] + s_c[6] * taud + (double)(((cy[4] * tau + cy[3]) * tau + cy[2]) * t2);
+                const double s2 = xd * xd + yd * yd;
+                x = (Real)xd;
+                y = (Real)yd;
+                z = d_sqrt((Real)s2);
+                u = (Real)(s2 - 1.0) / (Real(1) + z);
+            } else {
+                x = horner5(cx, tau);
+                y = horner5(cy, tau);
+                z = sqrt(x * x + y * y);
+                u = z - Real(1);
+            }
             if (!(z < zlim)) continue;
             const Real vx = dhorner5(cx, tau), vy = dhorner5(cy, tau);
             const Real px = horner5(dcx, tau), py = horner5(dcy, tau);
             // limb darkening and its slopes at g = z/(1+k)
-            const Real gg = z * inv1pk;
+            // the grazing parameter in units of the g step: in the mixed form from
+            // 1 + u in FP64, so the lerp position keeps the precision of u
+            Real gg, gx;
+            if constexpr (kMixed) {
+                const double gxd = (1.0 + (double)u) * g_scale;
+                gx = (Real)gxd;
+                gg = (Real)(gxd * s.dg);
+            } else {
+                gg = z * inv1pk;
+                gx = gg * inv_dg;
+            }
             Real Ii = 0, dIdg = 0, Ik = 0, Ic0 = 0, Ic1 = 0;
             if (gg <= Real(1)) {
-                int i = (int)floor(gg * inv_dg);
+                int i = (int)floor(gx);
                 i = i > s.ng - 2 ? s.ng - 2 : (i < 0 ? 0 : i);
-                const Real al = gg * inv_dg - (Real)i;
+                const Real al = gx - (Real)i;
                 Ii = s_L[i] + al * (s_L[i + 1] - s_L[i]);
                 dIdg = s_dL[i];
                 Ik = s_Lk[i] + al * (s_Lk[i + 1] - s_Lk[i]);
@@ -236,7 +299,12 @@ __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
                 A = pik2; dAdz = 0; dAdk = dpik2;
             } else {
                 Real kap, akite;
-                ccia_kite3<Real>(Real(1), kr, z, A, kap, akite);
+                if constexpr (kMixed) {
+                    if (kk < 1.0) lens_seg_f32(kr, z, u, A, kap, akite);
+                    else ccia_kite3<Real>(Real(1), kr, z, A, kap, akite);
+                } else {
+                    ccia_kite3<Real>(Real(1), kr, z, A, kap, akite);
+                }
                 dAdz = Real(-2) * akite / z;   // minus the chord of the lens
                 dAdk = Real(2) * kr * kap;
             }
