@@ -505,26 +505,29 @@ __device__ __forceinline__ void ts_store(Real* o, const Real* v, int n, bool vec
 // (c1 = A0 inside z <= 1 - k_max, else c2 = A0 and c3 = 2 k_mean kappa0, the
 // serial model's two branches); outside the window, and for g > 1 where both the
 // mean and the area vanish, F = 1 is selected, so a NaN row of limb-darkening
-// means reaches only the in-window values, as in the reference.  The LD rows of a chunk of passbands sit in LDS transposed,
-// passband fastest, so the rows of a 16-passband group are read with immediate
-// offsets from one address.  A wave none of whose times is in the window stores
-// ones without arithmetic.
+// means reaches only the in-window values, as in the reference.  The LD rows of a
+// chunk of passbands sit in LDS transposed, passband fastest, so the rows of a
+// 16-passband group are read with immediate offsets from one address; the row
+// stride is 65 values, so lanes reading different g hit different banks (a stride
+// of 64 would put every lane of a row read in one bank).  A wave none of whose
+// times is in the window stores ones without arithmetic.
 constexpr int kTsRowThreads = 1024;
 constexpr int kTsRowPc = 64;      // passbands per staged chunk
+constexpr int kTsRowLd = kTsRowPc + 1;   // LDS stride of one g
 constexpr int kTsRowQ = 16;       // passbands per unrolled group
 // time vectors (16 bytes: 4 FP32 or 2 FP64 times) per thread; npt <= 8192 (FP32), 6144 (FP64)
 template <typename Real> constexpr int ts_row_vec() { return sizeof(Real) == 4 ? 2 : 3; }
 template <typename Real> constexpr int ts_row_max_npt() { return kTsRowThreads * ts_row_vec<Real>() * ts_tpl<Real>(); }
 
-static inline size_t ts_rows_lds(int ng, size_t real) { return ((size_t)kTsRowPc * ng + 3 * kTsRowPc) * real; }
+static inline size_t ts_rows_lds(int ng, size_t real) { return ((size_t)kTsRowLd * ng + 3 * kTsRowPc) * real; }
 
 template <typename Real>
 __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_kernel(TsArgs s, Real* flux) {
     constexpr int TPL = ts_tpl<Real>();
     constexpr int NV = ts_row_vec<Real>();
     extern __shared__ __align__(16) unsigned char lds[];
-    Real* s_ldm = (Real*)lds;                        // [ng][kTsRowPc]
-    Real* s_afac = s_ldm + (size_t)kTsRowPc * s.ng;
+    Real* s_ldm = (Real*)lds;                        // [ng][kTsRowLd]
+    Real* s_afac = s_ldm + (size_t)kTsRowLd * s.ng;
     Real* s_kd = s_afac + kTsRowPc;
     Real* s_iist = s_kd + kTsRowPc;
 
@@ -574,7 +577,7 @@ __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_kernel(TsArgs s, R
                     any = true;
                     if (!g.gz) {
                         use[u][j] = true;
-                        off[u][j] = g.ig * kTsRowPc;
+                        off[u][j] = g.ig * kTsRowLd;
                         rb[u][j] = g.rb;
                         ra[u][j] = g.ra;
                         c1[u][j] = g.inside ? g.ap0 : Real(0);
@@ -590,9 +593,9 @@ __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_kernel(TsArgs s, R
         const int npc = min(kTsRowPc, s.npb - pb0);
         __syncthreads();  // the previous chunk's rows are no longer read
         for (int i = threadIdx.x; i < kTsRowPc * s.ng; i += kTsRowThreads) {
-            const int q = i / s.ng, g = i - q * s.ng;
+            const int q = i / s.ng, g = i - q * s.ng;   // coalesced global reads along g
             const size_t src = ((size_t)pv * s.npb + pb0 + q) * s.ng + g;
-            s_ldm[g * kTsRowPc + q] = q < npc ? (sizeof(Real) == 4 ? (Real)s.ldm32[src] : (Real)s.ldm64[src]) : Real(0);
+            s_ldm[g * kTsRowLd + q] = q < npc ? (sizeof(Real) == 4 ? (Real)s.ldm32[src] : (Real)s.ldm64[src]) : Real(0);
         }
         for (int q = threadIdx.x; q < npc; q += kTsRowThreads) {
             const double kq = s.k[(size_t)pv * s.npb + pb0 + q];
@@ -619,7 +622,7 @@ __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_kernel(TsArgs s, R
 #pragma unroll
                                 for (int j = 0; j < TPL; ++j) {
                                     const Real* l = lq + off[u][j] + qq;
-                                    const Real ip = rb[u][j] * l[0] + ra[u][j] * l[kTsRowPc];
+                                    const Real ip = rb[u][j] * l[0] + ra[u][j] * l[kTsRowLd];
                                     const Real area = c1[u][j] * afac + (c2[u][j] + c3[u][j] * kd);
                                     w[j] = use[u][j] ? Real(1) - (ip * area) * f : Real(1);
                                 }
