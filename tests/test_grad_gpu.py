@@ -111,3 +111,72 @@ def test_lnlike_invalid_and_out_of_table(gpu, case):
     lnl, g = gpu_lnl(wl, 'fp64', obs, sigma, th)
     assert np.isfinite(lnl[0]) and np.isnan(lnl[1]) and np.isnan(lnl[2])
     assert np.isnan(g[1]).all() and np.isnan(g[2]).all()
+
+
+# At the gradient benchmark's size (2048 pv x 10 000 t, nss = 5).  Two plain
+# relative checks on the components with |g| >= 0.3 x the parameter's RMS over
+# the population (about 3/4 of them; below that floor a component is a near-zero
+# difference of large per-point terms and its relative error says nothing):
+#   * the mixed-precision adjoint against the FP64 adjoint, bar 1e-4;
+#   * the FP64 adjoint against central differences of the FP64 lnL on the GPU
+#     (pinned to the oracle's lnL on a 16-pv slice here, and at rtol 1e-12 above)
+#     at h = 1e-8 on 256 pv.  At h = 1e-6 the secants straddle the limb-darkening
+#     lerp's knots often enough at this size to move t0 and i by up to 7e-4.
+FLOOR = 0.3
+
+
+@pytest.fixture(scope='module')
+def scale_case(gpu):
+    wl = workloads.rr_gradient(npv=2048)
+    tr = wl.truth
+    ref = RoadRunnerModel('quadratic', precision='fp64')
+    ref.set_data(wl.times, nsamples=wl.nsamples, exptimes=wl.exptimes)
+    f_true = ref.evaluate([[tr['k']]], [[[tr['u1'], tr['u2']]]], [[tr['t0']]], tr['p'], tr['a'], tr['i'])
+    obs = f_true + wl.noise_draw
+    theta = np.stack([wl.pop[n] for n in NAMES], 1)
+    return wl, obs, theta
+
+
+def plain_rel_above_floor(g, g_ref):
+    scale = np.sqrt((g_ref ** 2).mean(0))
+    keep = np.abs(g_ref) >= FLOOR * scale
+    rel = np.where(keep, np.abs(g - g_ref) / np.where(keep, np.abs(g_ref), 1.0), 0.0)
+    return rel.max(0), keep.mean(0)
+
+
+def test_mixed_adjoint_vs_fp64_at_scale(gpu, scale_case):
+    wl, obs, theta = scale_case
+    say(f'case: mixed vs FP64 adjoint, {theta.shape[0]} pv x {wl.times.size} t, nss=5')
+    lnl_m, g_m = gpu_lnl(wl, 'mixed', obs, wl.noise, theta)
+    lnl_d, g_d = gpu_lnl(wl, 'fp64', obs, wl.noise, theta)
+    assert np.isfinite(g_d).all() and np.isfinite(g_m).all()
+    rel, kept = plain_rel_above_floor(g_m, g_d)
+    for j, n in enumerate(NAMES):
+        say(f'  {n:3s} max plain rel {rel[j]:.2e} over {kept[j]:.0%} of the pv')
+    assert (kept > 0.5).all()
+    assert rel.max() <= 1e-4
+    np.testing.assert_allclose(lnl_m, lnl_d, rtol=1e-6)
+
+
+def test_fp64_adjoint_vs_finite_differences_at_scale(gpu, scale_case, oracle):
+    wl, obs, theta = scale_case
+    th = theta[:256]
+    say(f'case: FP64 adjoint vs central differences (h = 1e-8), 256 pv x {wl.times.size} t, nss=5')
+    lnl, g = gpu_lnl(wl, 'fp64', obs, wl.noise, th)
+    # the differentiated function is the oracle's lnL
+    om = oracle.OracleRR('quadratic')
+    om.set_data(wl.times, nsamples=wl.nsamples, exptimes=wl.exptimes)
+    np.testing.assert_allclose(lnl[:16], oracle_lnl(om, obs, wl.noise, th[:16]), rtol=1e-12)
+    h = 1e-8
+    g_fd = np.zeros_like(th)
+    for j in range(th.shape[1]):
+        tp, tm = th.copy(), th.copy()
+        tp[:, j] += h
+        tm[:, j] -= h
+        g_fd[:, j] = (gpu_lnl(wl, 'fp64', obs, wl.noise, tp, grad=False) -
+                      gpu_lnl(wl, 'fp64', obs, wl.noise, tm, grad=False)) / (2 * h)
+    rel, kept = plain_rel_above_floor(g, g_fd)
+    for j, n in enumerate(NAMES):
+        say(f'  {n:3s} max plain rel {rel[j]:.2e} over {kept[j]:.0%} of the pv')
+    assert (kept > 0.5).all()
+    assert rel.max() <= 1e-4
