@@ -309,6 +309,34 @@ def test_short_periods_window_search(gpu, oracle):
         assert err <= tol
 
 
+@pytest.mark.parametrize('case', ['bjd', 'fast'])
+def test_samples_beyond_contacts(gpu, oracle, case):
+    # One light curve with one nsamples.  A point in the window whose samples all
+    # lie outside the disk, like a point outside the window, must come out exactly
+    # 1 in both paths wherever the oracle gives exactly 1 (the FP32 depth sum is 0,
+    # the FP64 sum is ns ones).  'fast': periods of 0.3-0.6 d with a 1-h exposure,
+    # where a sample reaches 0.09 d past a contact.
+    rng = np.random.default_rng(5)
+    npv = 32
+    wl = workloads.rr_bjd(npv=npv, npt=6000, seed=5)
+    if case == 'fast':
+        wl.nsamples, wl.exptimes = 4, 1.0 / 24
+        wl.p = rng.uniform(0.3, 0.6, npv)
+        wl.a = workloads.as_from_rhop(rng.uniform(1.0, 3.0, npv), wl.p)
+        wl.i = workloads.i_from_ba(rng.uniform(0.0, 0.95, npv), wl.a)
+        wl.t0 = (wl.times[0] + rng.uniform(0.0, 1.0, npv))[:, None]
+    say(f'case: samples beyond the contacts, {case}: {npv} pv x 6000 t, nss={wl.nsamples}')
+    fo = oracle_flux(oracle, wl, 'power-2')
+    ones = fo == 1.0
+    assert ones.sum() > 1000 and (fo < 1).sum() > 1000
+    for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
+        fg = gpu_flux(wl, prec)
+        err = np.nanmax(np.abs(fg - fo))
+        say(f'  {prec}: max |dF| = {err:.3e}, non-ones where the oracle has ones: {(fg[ones] != 1.0).sum()}')
+        assert (fg[ones] == 1.0).all()
+        assert err <= tol
+
+
 def test_k_outside_table_uses_exact_weights(gpu, oracle):
     say('case: k outside klims -> exact 2-D weights')
     wl = workloads.rr_small(npv=8)
