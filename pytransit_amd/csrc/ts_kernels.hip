@@ -10,8 +10,10 @@
 //
 // Kernels:
 //   ts_setup_kernel    one 64-lane workgroup per pv (orbit on lanes 0-6, disk integrals)
-//   ts_weights_kernel  one lane per (pv, g): the 2-D weight row at k_mean, written
-//                      transposed W2T[pv][z][g] (FP64 and FP32, g padded to 16)
+//   ts_weights_kernel  up to 16 (pv, g) rows per workgroup: the 2-D weight rows at
+//                      k_mean, the lens areas that need arccos evaluated from a
+//                      list on full waves, written transposed W2T[pv][z][g] (FP64
+//                      and FP32, g padded to 16)
 //   ts_ldm_tab_mfma_kernel  FP32 MFMA contraction ldm = ldp . W2T for tabulated FP64
 //                      profiles: one 8-wave workgroup per pv, W2T in LDS, each row
 //                      read as 128-byte pieces (16-byte loads, a permuted z order)
@@ -61,22 +63,35 @@ struct TsArgs {
 constexpr int kTsGroup = 8;
 
 // ---- per-pv setup; model_trspec.py:36-68 ----------------------------------------------
+constexpr int kTsKChunk = 512;   // radius ratios of a pv staged in LDS at a time
+
 __global__ void __launch_bounds__(64) ts_setup_kernel(TsArgs s) {
     __shared__ double s_x[8][8], s_y[8][8], s_c[8][10], s_w[8][2], s_km[8];
+    __shared__ double s_kb[kTsGroup][kTsKChunk];
     const int sub = threadIdx.x / kTsGroup, j = threadIdx.x % kTsGroup;
     const int pv = blockIdx.x * kTsGroup + sub;
     const bool inrange = pv < s.npv;
     const int q = inrange ? pv : 0;
     const double a = s.a[q], e = s.e[q];
     const bool valid = inrange && !(isnan(a) || a <= 1.0 || e < 0.0);
+    // mean as numba's left-to-right sum, and the max, of the pv's row of radius
+    // ratios: the row is loaded into LDS by the pv's 8 lanes (all loads in flight
+    // at once), then summed in order by one lane
+    double sum = 0.0, kmx = -INFINITY;
+    const double* kr = s.k + (size_t)q * s.npb;
+    for (int c0 = 0; c0 < s.npb; c0 += kTsKChunk) {
+        const int cn = min(kTsKChunk, s.npb - c0);
+        if (inrange)
+            for (int i = j; i < cn; i += kTsGroup) s_kb[sub][i] = kr[c0 + i];
+        __syncthreads();
+        if (inrange && j == 0)
+            for (int i = 0; i < cn; ++i) {
+                sum += s_kb[sub][i];
+                kmx = fmax(kmx, s_kb[sub][i]);
+            }
+        __syncthreads();
+    }
     if (inrange && j == 0) {
-        // mean as numba's left-to-right sum; max of the row
-        double sum = 0.0, kmx = -INFINITY;
-        const double* kr = s.k + (size_t)q * s.npb;
-        for (int i = 0; i < s.npb; ++i) {
-            sum += kr[i];
-            kmx = fmax(kmx, kr[i]);
-        }
         s_km[sub] = sum / s.npb;
         TsRec* r = s.rec + q;
         r->kmean = sum / s.npb;
@@ -139,56 +154,111 @@ __global__ void __launch_bounds__(256) ts_istar_kernel(TsArgs s) {
 }
 
 // ---- weights at k_mean, transposed: W2T[pv][z][g] --------------------------------------
+// A workgroup takes R rows (g values) of one pv.  The exact row
+// (calculate_weights_2d, common.py:152-185) is the differences of the lens areas
+// at consecutive z edges over their sum.  The areas go to LDS: the cheap cases of
+// the lens area (no overlap, one disk inside the other) at once, the (r, z) pairs
+// that need the arccos form to a list that all lanes then work through, so no lane
+// idles in a wave that evaluates it.  The sum runs over the differences in z order
+// as in the reference; the rows' values for one z are written next to each other.
+constexpr int kTwMaxRows = 16;
+__host__ __device__ inline int ts_weight_rows(int nz) {
+    const int r = 2048 / (nz > 0 ? nz : 1);
+    return r < 1 ? 1 : (r > kTwMaxRows ? kTwMaxRows : r);
+}
+// LDS of the exact-weight kernel: areas [R][nz], the list [R][nz] (16-bit, padded
+// to 8 bytes), differences [R][nz]
+__host__ __device__ inline size_t ts_list_bytes(int nz) {
+    return ((size_t)ts_weight_rows(nz) * nz * sizeof(uint16_t) + 7) / 8 * 8;
+}
+__host__ __device__ inline size_t ts_weights_lds(int nz) {
+    return (size_t)ts_weight_rows(nz) * nz * 2 * sizeof(double) + ts_list_bytes(nz);
+}
+
 __global__ void __launch_bounds__(256) ts_weights_kernel(TsArgs s) {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)s.npv * s.ngp) return;
-    const int pv = (int)(idx / s.ngp), ig = (int)(idx - (long long)pv * s.ngp);
-    const size_t base = (size_t)pv * s.nzp * s.ngp + ig;
+    extern __shared__ __align__(16) double s_a[];   // [R][nz] lens areas, the list, the differences
+    __shared__ int s_n;
+    __shared__ double s_sum[kTwMaxRows];
+    const int R = ts_weight_rows(s.nz), nz = s.nz;
+    const int nchunk = (s.ngp + R - 1) / R;
+    const int pv = blockIdx.x / nchunk;
+    const int g0 = (blockIdx.x - pv * nchunk) * R;
+    const int nr = min(R, s.ngp - g0);
+    const int tid = threadIdx.x;
+    const size_t base = (size_t)pv * s.nzp * s.ngp + g0;
+    auto put = [&](int iz, int row, double v) {
+        if (s.w2_64) s.w2_64[base + (size_t)iz * s.ngp + row] = v;
+        if (s.w2_32) s.w2_32[base + (size_t)iz * s.ngp + row] = (float)v;
+    };
     const TsRec& r = s.rec[pv];
-    if (ig >= s.ng) {
-        for (int iz = 0; iz < s.nzp; ++iz) {
-            if (s.w2_64) s.w2_64[base + (size_t)iz * s.ngp] = 0.0;
-            if (s.w2_32) s.w2_32[base + (size_t)iz * s.ngp] = 0.0f;
+    if (r.ik >= 0) {
+        // (1-ak) W[ik] + ak W[ik+1] (device table layout [nk][nz][ng])
+        const double* w0 = s.W + (size_t)r.ik * nz * s.ng;
+        const double* w1 = w0 + (size_t)nz * s.ng;
+        for (int i = tid; i < s.nzp * nr; i += blockDim.x) {
+            const int iz = i / nr, row = i - iz * nr, ig = g0 + row;
+            double v = 0.0;
+            if (iz < nz && ig < s.ng) v = (1.0 - r.ak) * w0[(size_t)iz * s.ng + ig] + r.ak * w1[(size_t)iz * s.ng + ig];
+            put(iz, row, v);
         }
         return;
     }
+    uint16_t* s_list = (uint16_t*)(s_a + (size_t)R * nz);
+    __shared__ double s_b[kTwMaxRows];
     const double km = r.kmean;
-    if (r.ik >= 0) {
-        // (1-ak) W[ik] + ak W[ik+1] (device table layout [nk][nz][ng])
-        const double* w0 = s.W + (size_t)r.ik * s.nz * s.ng + ig;
-        const double* w1 = w0 + (size_t)s.nz * s.ng;
-        for (int iz = 0; iz < s.nz; ++iz) {
-            const double v = (1.0 - r.ak) * w0[(size_t)iz * s.ng] + r.ak * w1[(size_t)iz * s.ng];
-            if (s.w2_64) s.w2_64[base + (size_t)iz * s.ngp] = v;
-            if (s.w2_32) s.w2_32[base + (size_t)iz * s.ngp] = (float)v;
+    if (tid == 0) s_n = 0;
+    if (tid < nr) s_b[tid] = linspace_at(0.0, 1.0 - 1e-7, s.ng, g0 + tid) * (1.0 + km);
+    __syncthreads();
+    auto b_of = [&](int ig) { return s_b[ig - g0]; };
+    const int lane = tid % 64;
+    for (int i0 = tid - lane; i0 < nr * nz; i0 += blockDim.x) {   // wave-uniform trip count
+        const int i = i0 + lane;
+        const int row = i / nz, iz = i - row * nz;
+        bool listed = false;
+        if (i < nr * nz && g0 + row < s.ng) {
+            const double b = b_of(g0 + row), r1 = s.ze[iz];
+            if (r1 < b - km || r1 >= b + km || b - km <= -r1) s_a[i] = cci_area(r1, km, b);   // cci_area's cheap cases
+            else listed = true;
         }
-    } else {
-        // calculate_weights_2d(k_mean); common.py:152-185
-        const double g = linspace_at(0.0, 1.0 - 1e-7, s.ng, ig);
-        const double b = g * (1.0 + km);
-        double a0 = cci_area(s.ze[0], km, b), sum = a0;
-        for (int iz = 1; iz < s.nz; ++iz) {
-            const double a1 = cci_area(s.ze[iz], km, b);
-            sum += a1 - a0;
-            a0 = a1;
-        }
-        a0 = cci_area(s.ze[0], km, b);
-        for (int iz = 0; iz < s.nz; ++iz) {
-            double v;
-            if (iz == 0) {
-                v = a0 / sum;
-            } else {
-                const double a1 = cci_area(s.ze[iz], km, b);
-                v = (a1 - a0) / sum;
-                a0 = a1;
-            }
-            if (s.w2_64) s.w2_64[base + (size_t)iz * s.ngp] = v;
-            if (s.w2_32) s.w2_32[base + (size_t)iz * s.ngp] = (float)v;
+        // one LDS atomic per wave for its listed pairs
+        const unsigned long long m = __ballot(listed);
+        if (m) {
+            const int first = __ffsll((long long)m) - 1;
+            int at = 0;
+            if (lane == first) at = atomicAdd(&s_n, __popcll(m));
+            at = __shfl(at, first, 64);
+            if (listed) s_list[at + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)i;
         }
     }
-    for (int iz = s.nz; iz < s.nzp; ++iz) {
-        if (s.w2_64) s.w2_64[base + (size_t)iz * s.ngp] = 0.0;
-        if (s.w2_32) s.w2_32[base + (size_t)iz * s.ngp] = 0.0f;
+    __syncthreads();
+    const int nl = s_n;
+    for (int j = tid; j < nl; j += blockDim.x) {
+        const int i = s_list[j], row = i / nz, iz = i - row * nz;
+        s_a[i] = cci_area(s.ze[iz], km, b_of(g0 + row));
+    }
+    __syncthreads();
+    // the differences, all at once, then each row's sum over them in z order
+    double* s_d = (double*)((unsigned char*)s_list + ts_list_bytes(nz));
+    for (int i = tid; i < nr * nz; i += blockDim.x) {
+        const int iz = i % nz;
+        s_d[i] = iz == 0 ? s_a[i] : s_a[i] - s_a[i - 1];
+    }
+    __syncthreads();
+    if (tid < nr) {
+        double sum = 0.0;
+        if (g0 + tid < s.ng) {
+            const double* d = s_d + (size_t)tid * nz;
+#pragma unroll 8
+            for (int iz = 0; iz < nz; ++iz) sum += d[iz];
+        }
+        s_sum[tid] = sum;
+    }
+    __syncthreads();
+    for (int i = tid; i < s.nzp * nr; i += blockDim.x) {
+        const int iz = i / nr, row = i - iz * nr;
+        double v = 0.0;
+        if (iz < nz && g0 + row < s.ng) v = s_d[(size_t)row * nz + iz] / s_sum[row];
+        put(iz, row, v);
     }
 }
 
@@ -519,7 +589,7 @@ constexpr int kTsRowQ = 16;       // passbands per unrolled group
 template <typename Real> constexpr int ts_row_vec() { return sizeof(Real) == 4 ? 2 : 3; }
 template <typename Real> constexpr int ts_row_max_npt() { return kTsRowThreads * ts_row_vec<Real>() * ts_tpl<Real>(); }
 
-static inline size_t ts_rows_lds(int ng, size_t real) { return ((size_t)kTsRowLd * ng + 3 * kTsRowPc) * real; }
+__host__ __device__ inline size_t ts_rows_lds(int ng, size_t real) { return ((size_t)kTsRowLd * ng + 3 * kTsRowPc) * real; }
 
 template <typename Real>
 __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_kernel(TsArgs s, Real* flux) {
@@ -631,6 +701,171 @@ __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_kernel(TsArgs s, R
                                 ts_store(o + (size_t)v * TPL, vone, TPL, true);
                             }
                         }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---- FP32 rows in whole cache lines ----------------------------------------------------
+//
+// A row of npt FP32 values starts 4 (R npt mod 32) bytes past a 128-byte boundary
+// (R the row's index in the flux array).  In the row form above, thread i owns the
+// same time vectors in every row, so in a row that starts off a boundary every
+// wave's 1 KB store straddles cache lines, and HBM takes those pieces at ~4.0
+// TB/s against ~4.9 TB/s for whole lines (the same kernel at npt = 5000 and 4992).
+// Here the geometry of every time is reduced once to its row coefficients in an
+// LDS table, one array per coefficient, and in each row slot s takes time vector
+// (s + head) mod nv, head being the number of 16-byte vectors before the row's
+// first 128-byte boundary: the 64 slots of a wave then cover one aligned
+// kilobyte.  Rows P = 32 / gcd(npt, 32) apart have the same head and share one
+// load of the coefficients from the table into registers; the per-value
+// arithmetic is the row form's.
+constexpr int kTsLine = 32;   // FP32 values per 128-byte cache line
+constexpr size_t kTsRowsAlignedMaxLds = 160 * 1024;
+
+__host__ __device__ inline size_t ts_rows_table_offset(int ng) { return (ts_rows_lds(ng, 4) + 15) / 16 * 16; }
+__host__ __device__ inline size_t ts_rows_aligned_lds(int ng, int npt) { return ts_rows_table_offset(ng) + (size_t)npt * 5 * 4; }
+
+__global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_aligned_kernel(TsArgs s, float* flux) {
+    using Real = float;
+    constexpr int TPL = ts_tpl<float>();
+    constexpr int NV = ts_row_vec<float>();
+    extern __shared__ __align__(16) unsigned char lds[];
+    Real* s_ldm = (Real*)lds;                        // [ng][kTsRowLd]
+    Real* s_afac = s_ldm + (size_t)kTsRowLd * s.ng;
+    Real* s_kd = s_afac + kTsRowPc;
+    Real* s_iist = s_kd + kTsRowPc;
+    const int npt = s.npt, nv = npt / TPL;
+    // the row coefficients of every time: LDS offset << 2 | inside << 1 | use, and
+    // the lerp weights, the area at k_mean and its k-derivative
+    int* t_pk = (int*)(lds + ts_rows_table_offset(s.ng));
+    Real* t_ra = (Real*)(t_pk + npt);
+    Real* t_rb = t_ra + npt;
+    Real* t_a0 = t_rb + npt;
+    Real* t_dk = t_a0 + npt;
+
+    const int pv = xcd_remap(blockIdx.x, gridDim.x);
+    const TsRec& r = s.rec[pv];
+    float* out = flux + (size_t)pv * s.npb * npt;
+    if (!r.good) {
+        const f32x4_t vnan = {NAN, NAN, NAN, NAN};
+        for (size_t i = threadIdx.x; i < (size_t)s.npb * nv; i += kTsRowThreads)
+            __builtin_nontemporal_store(vnan, (f32x4_t*)(out + i * TPL));
+        return;
+    }
+    const double ext = s.exptimes[0];
+    const double t0v = s.t0[pv];
+    for (int t = threadIdx.x; t < npt; t += kTsRowThreads) {
+        double tc;
+        {
+#pragma clang fp contract(off)
+            const double tt = s.times[t];
+            const double epoch = floor((tt - t0v + 0.5 * r.p) * r.inv_p);
+            const double uu = tt - t0v + 0.5 * r.p;
+            const double rem = uu - epoch * r.p;
+            const double ep = epoch + ((rem < 0.0) ? -1.0 : ((rem >= r.p) ? 1.0 : 0.0));
+            tc = tt - (t0v + ep * r.p);
+        }
+        int pk = 0;
+        Real ra = Real(0), rb = Real(0), a0 = Real(0), dk = Real(0);
+        if ((r.bt1 <= tc) && (tc <= r.bt4)) {
+            const TsGeo<Real> g = ts_geometry<Real>(r, tc, ext, 1, 0, s.dg, s.ng);
+            if (!g.gz) {
+                pk = ((g.ig * kTsRowLd) << 2) | (g.inside ? 2 : 0) | 1;
+                ra = g.ra;
+                rb = g.rb;
+                a0 = g.ap0;
+                dk = g.dadk;
+            }
+        }
+        t_pk[t] = pk;
+        t_ra[t] = ra;
+        t_rb[t] = rb;
+        t_a0[t] = a0;
+        t_dk[t] = dk;
+    }
+    // rows P apart start equally far past a 128-byte boundary
+    const int gl = npt & -npt;   // lowest set bit of npt
+    const int P = kTsLine / (gl < kTsLine ? gl : kTsLine);
+    for (int pb0 = 0; pb0 < s.npb; pb0 += kTsRowPc) {
+        const int npc = min(kTsRowPc, s.npb - pb0);
+        __syncthreads();  // the previous chunk's rows are no longer read (and the table is written)
+        for (int i = threadIdx.x; i < kTsRowPc * s.ng; i += kTsRowThreads) {
+            const int q = i / s.ng, g = i - q * s.ng;   // coalesced global reads along g
+            const size_t src = ((size_t)pv * s.npb + pb0 + q) * s.ng + g;
+            s_ldm[g * kTsRowLd + q] = q < npc ? s.ldm32[src] : Real(0);
+        }
+        for (int q = threadIdx.x; q < npc; q += kTsRowThreads) {
+            const double kq = s.k[(size_t)pv * s.npb + pb0 + q];
+            s_afac[q] = (Real)(kq * kq / (r.kmean * r.kmean));
+            s_kd[q] = (Real)(kq - r.kmean);
+            s_iist[q] = (Real)(1.0 / s.istar[(size_t)pv * s.npb + pb0 + q]);
+        }
+        __syncthreads();
+        for (int q0 = pb0; q0 < pb0 + min(P, npc); ++q0) {
+            // this class's head, and each slot's coefficients from the table
+            const int mis = (int)(((uintptr_t)(out + (size_t)q0 * npt) >> 2) & (kTsLine - 1));
+            const int head = ((kTsLine - mis) & (kTsLine - 1)) / TPL;
+            int vi[NV], off[NV][TPL];
+            Real rb[NV][TPL], ra[NV][TPL], c1[NV][TPL], c2[NV][TPL], c3[NV][TPL];
+            bool use[NV][TPL], act[NV];
+#pragma unroll
+            for (int u = 0; u < NV; ++u) {
+                const int slot = threadIdx.x + u * kTsRowThreads;
+                vi[u] = slot + head < nv ? slot + head : slot + head - nv;
+                bool any = false;
+                if (slot < nv) {
+                    const int4 pk4 = *(const int4*)(t_pk + TPL * vi[u]);
+                    const f32x4_t ra4 = *(const f32x4_t*)(t_ra + TPL * vi[u]);
+                    const f32x4_t rb4 = *(const f32x4_t*)(t_rb + TPL * vi[u]);
+                    const f32x4_t a04 = *(const f32x4_t*)(t_a0 + TPL * vi[u]);
+                    const f32x4_t dk4 = *(const f32x4_t*)(t_dk + TPL * vi[u]);
+                    const int pk[TPL] = {pk4.x, pk4.y, pk4.z, pk4.w};
+#pragma unroll
+                    for (int j = 0; j < TPL; ++j) {
+                        const bool inside = (pk[j] & 2) != 0;
+                        use[u][j] = (pk[j] & 1) != 0;
+                        off[u][j] = pk[j] >> 2;
+                        ra[u][j] = ra4[j];
+                        rb[u][j] = rb4[j];
+                        c1[u][j] = inside ? a04[j] : Real(0);
+                        c2[u][j] = inside ? Real(0) : a04[j];
+                        c3[u][j] = inside ? Real(0) : dk4[j];
+                        any = any || use[u][j];
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < TPL; ++j) {
+                        use[u][j] = false;
+                        off[u][j] = 0;
+                        ra[u][j] = rb[u][j] = c1[u][j] = c2[u][j] = c3[u][j] = Real(0);
+                    }
+                }
+                act[u] = __ballot(any) != 0ull;
+            }
+#pragma unroll 2
+            for (int q = q0; q < pb0 + npc; q += P) {
+                const int ql = q - pb0;
+                const Real f = s_iist[ql], afac = s_afac[ql], kd = s_kd[ql];
+                const Real* lq = s_ldm + ql;
+                float* o = out + (size_t)q * npt;
+#pragma unroll
+                for (int u = 0; u < NV; ++u) {
+                    const int slot = threadIdx.x + u * kTsRowThreads;
+                    if (slot < nv) {
+                        f32x4_t w = {1.0f, 1.0f, 1.0f, 1.0f};
+                        if (act[u]) {
+#pragma unroll
+                            for (int j = 0; j < TPL; ++j) {
+                                const Real* l = lq + off[u][j];
+                                const Real ip = rb[u][j] * l[0] + ra[u][j] * l[kTsRowLd];
+                                const Real area = c1[u][j] * afac + (c2[u][j] + c3[u][j] * kd);
+                                w[j] = use[u][j] ? Real(1) - (ip * area) * f : Real(1);
+                            }
+                        }
+                        __builtin_nontemporal_store(w, (f32x4_t*)(o + (size_t)vi[u] * TPL));
                     }
                 }
             }
@@ -838,14 +1073,17 @@ int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
     a.kmin = tab->kmin; a.kmax_tab = tab->kmax; a.dk = tab->dk; a.dg = tab->dg;
     hipStream_t st = (hipStream_t)stream;
     const bool tab_ld = ld->law == RR_LD_TABULATED;
+    // the weight kernel lists (g, z) pairs of a workgroup as 16-bit indices
+    if ((size_t)ts_weight_rows(a.nz) * a.nz > 65535u) return set_error(RR_ERANGE, "ts_evaluate: nz must be <= 65535");
 
     hipLaunchKernelGGL(ts_setup_kernel, dim3((npv + kTsGroup - 1) / kTsGroup), dim3(64), 0, st, a);
     int rc = check_launch("ts_setup_kernel");
     if (rc) return rc;
     hipLaunchKernelGGL(ts_istar_kernel, dim3((npv * npb + 255) / 256), dim3(256), 0, st, a);
     if ((rc = check_launch("ts_istar_kernel"))) return rc;
-    const long long nw = (long long)npv * a.ngp;
-    hipLaunchKernelGGL(ts_weights_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, a);
+    const int wrows = ts_weight_rows(a.nz);
+    const long long nw = (long long)npv * ((a.ngp + wrows - 1) / wrows);
+    hipLaunchKernelGGL(ts_weights_kernel, dim3((unsigned)nw), dim3(256), ts_weights_lds(a.nz), st, a);
     if ((rc = check_launch("ts_weights_kernel"))) return rc;
     // tabulated FP64 profiles with nz a multiple of 4 (16-byte aligned rows), nz <= 128
     // and ngp <= 128: the row-contiguous kernel
@@ -896,6 +1134,12 @@ int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
     const int uns = lc->uniform_nsamples;
     // one sample, rows of whole 16-byte vectors: the row form
     if (uns == 1 && ((uintptr_t)flux % 16) == 0) {
+        if (mixed && npt % 4 == 0 && npt <= ts_row_max_npt<float>() &&
+            ts_rows_aligned_lds(a.ng, npt) <= kTsRowsAlignedMaxLds) {
+            hipLaunchKernelGGL(ts_flux_rows_aligned_kernel, dim3((unsigned)npv), dim3(kTsRowThreads),
+                               ts_rows_aligned_lds(a.ng, npt), st, a, (float*)flux);
+            return check_launch("ts_flux_rows_aligned_kernel");
+        }
         if (mixed && npt % 4 == 0 && npt <= ts_row_max_npt<float>() && ts_rows_lds(a.ng, 4) <= 64 * 1024) {
             hipLaunchKernelGGL(ts_flux_rows_kernel<float>, dim3((unsigned)npv), dim3(kTsRowThreads),
                                ts_rows_lds(a.ng, 4), st, a, (float*)flux);
