@@ -85,6 +85,7 @@ __global__ void __launch_bounds__(64) ts_setup_kernel(TsArgs s) {
             for (int i = j; i < cn; i += kTsGroup) s_kb[sub][i] = kr[c0 + i];
         __syncthreads();
         if (inrange && j == 0)
+#pragma unroll 8
             for (int i = 0; i < cn; ++i) {
                 sum += s_kb[sub][i];
                 kmx = fmax(kmx, s_kb[sub][i]);
