@@ -1574,9 +1574,27 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     }
 }
 
+// Work items of the XCD-aware order go through the pvs in groups of kTileGroup:
+// within a group, the same time tile of every pv before the next tile.  Measured at
+// 4096 pv x 20 000 t against the pv-major order (each pv's tiles in a row): FP32
+// 102-103 -> 96-98 us, FP64 258 -> 237 us, the same HBM reads.  A tile-major order
+// over all 4096 pvs gained less (FP64 248 us) and read 4.5x more (each XCD then
+// touches every pv's record and limb-darkening row).
+constexpr int kTileGroup = 256;
+
+// (pv, tile) block index of work item `item` of `nitems` = npv x ntiles.
+__device__ __forceinline__ int grouped_tile(int item, int nitems, int ntiles) {
+    const int npv = nitems / ntiles;
+    const int per = kTileGroup * ntiles;
+    const int grp = item / per, rem = item - grp * per;
+    const int gsz = min(kTileGroup, npv - grp * kTileGroup);   // the last group may be smaller
+    const int tile = rem / gsz, pv = grp * kTileGroup + (rem - tile * gsz);
+    return pv * ntiles + tile;
+}
+
 template <bool kMixed, bool kFast>
 __global__ void __launch_bounds__(kThreads) rr_flux_kernel(FluxArgs f) {
-    flux_body<kMixed, kFast>(f, xcd_remap(blockIdx.x, gridDim.x));
+    flux_body<kMixed, kFast>(f, grouped_tile(xcd_remap(blockIdx.x, gridDim.x), gridDim.x, f.ntiles));
 }
 
 struct FluxBatch {
@@ -1589,7 +1607,7 @@ template <bool kMixed, bool kFast>
 __global__ void __launch_bounds__(kThreads) rr_flux_batch_kernel(FluxBatch b) {
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int i = batch_item(b.start, b.n, bid);
-    flux_body<kMixed, kFast>(b.a[i], bid - b.start[i]);
+    flux_body<kMixed, kFast>(b.a[i], grouped_tile(bid - b.start[i], b.start[i + 1] - b.start[i], b.a[i].ntiles));
 }
 
 // Device-memory batch (see rr_setup_batch_dev_kernel).  The XCD remap is over the
@@ -1603,7 +1621,7 @@ __global__ void __launch_bounds__(kThreads) rr_flux_batch_dev_kernel(const FluxB
     if ((int)blockIdx.x >= total) return;
     const int bid = xcd_remap(blockIdx.x, total);
     const int i = batch_item(b->start, n, bid);
-    flux_body<kMixed, kFast>(b->a[i], bid - b->start[i]);
+    flux_body<kMixed, kFast>(b->a[i], grouped_tile(bid - b->start[i], b->start[i + 1] - b->start[i], b->a[i].ntiles));
 }
 
 // A batch as the replayable launch reads it from device memory.
