@@ -724,6 +724,7 @@ __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_kernel(TsArgs s, R
 // load of the coefficients from the table into registers; the per-value
 // arithmetic is the row form's.
 constexpr int kTsLine = 32;   // FP32 values per 128-byte cache line
+constexpr int kTsStageRegs = 8;   // LD values per thread of a staged chunk: kTsRowPc x ng <= 8 x 1024
 constexpr size_t kTsRowsAlignedMaxLds = 160 * 1024;
 
 __host__ __device__ inline size_t ts_rows_table_offset(int ng) { return (ts_rows_lds(ng, 4) + 15) / 16 * 16; }
@@ -756,6 +757,38 @@ __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_aligned_kernel(TsA
             __builtin_nontemporal_store(vnan, (f32x4_t*)(out + i * TPL));
         return;
     }
+    // a chunk's LD rows and per-passband k and disk integral, loaded into registers
+    // one chunk ahead (every load of a chunk in flight at once, the wait for them
+    // after the previous chunk's rows) and stored to LDS after the chunk barrier
+    float st[kTsStageRegs];
+    double kq = 0.0, isq = 1.0;
+    auto stage_load = [&](int pb0) {
+        const int npc = min(kTsRowPc, s.npb - pb0);
+#pragma unroll
+        for (int m = 0; m < kTsStageRegs; ++m) {
+            const int i = threadIdx.x + m * kTsRowThreads;
+            const int q = i / s.ng, g = i - q * s.ng;
+            st[m] = (i < kTsRowPc * s.ng && q < npc) ? s.ldm32[((size_t)pv * s.npb + pb0 + q) * s.ng + g] : 0.0f;
+        }
+        if ((int)threadIdx.x < npc) {
+            kq = s.k[(size_t)pv * s.npb + pb0 + threadIdx.x];
+            isq = s.istar[(size_t)pv * s.npb + pb0 + threadIdx.x];
+        }
+    };
+    auto stage_store = [&](int pb0) {
+        const int npc = min(kTsRowPc, s.npb - pb0);
+#pragma unroll
+        for (int m = 0; m < kTsStageRegs; ++m) {
+            const int i = threadIdx.x + m * kTsRowThreads;
+            const int q = i / s.ng, g = i - q * s.ng;
+            if (i < kTsRowPc * s.ng) s_ldm[g * kTsRowLd + q] = st[m];
+        }
+        if ((int)threadIdx.x < npc) {
+            s_afac[threadIdx.x] = (Real)(kq * kq / (r.kmean * r.kmean));
+            s_kd[threadIdx.x] = (Real)(kq - r.kmean);
+            s_iist[threadIdx.x] = (Real)(1.0 / isq);
+        }
+    };
     const double ext = s.exptimes[0];
     const double t0v = s.t0[pv];
     for (int t = threadIdx.x; t < npt; t += kTsRowThreads) {
@@ -787,36 +820,38 @@ __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_aligned_kernel(TsA
         t_a0[t] = a0;
         t_dk[t] = dk;
     }
+    stage_load(0);
     // rows P apart start equally far past a 128-byte boundary
     const int gl = npt & -npt;   // lowest set bit of npt
     const int P = kTsLine / (gl < kTsLine ? gl : kTsLine);
     for (int pb0 = 0; pb0 < s.npb; pb0 += kTsRowPc) {
         const int npc = min(kTsRowPc, s.npb - pb0);
         __syncthreads();  // the previous chunk's rows are no longer read (and the table is written)
-        for (int i = threadIdx.x; i < kTsRowPc * s.ng; i += kTsRowThreads) {
-            const int q = i / s.ng, g = i - q * s.ng;   // coalesced global reads along g
-            const size_t src = ((size_t)pv * s.npb + pb0 + q) * s.ng + g;
-            s_ldm[g * kTsRowLd + q] = q < npc ? s.ldm32[src] : Real(0);
-        }
-        for (int q = threadIdx.x; q < npc; q += kTsRowThreads) {
-            const double kq = s.k[(size_t)pv * s.npb + pb0 + q];
-            s_afac[q] = (Real)(kq * kq / (r.kmean * r.kmean));
-            s_kd[q] = (Real)(kq - r.kmean);
-            s_iist[q] = (Real)(1.0 / s.istar[(size_t)pv * s.npb + pb0 + q]);
-        }
+        stage_store(pb0);
         __syncthreads();
+        if (pb0 + kTsRowPc < s.npb) stage_load(pb0 + kTsRowPc);
         for (int q0 = pb0; q0 < pb0 + min(P, npc); ++q0) {
             // this class's head, and each slot's coefficients from the table
             const int mis = (int)(((uintptr_t)(out + (size_t)q0 * npt) >> 2) & (kTsLine - 1));
             const int head = ((kTsLine - mis) & (kTsLine - 1)) / TPL;
+            // per value: LDS offset, lerp weights, the area at k_mean (negated when
+            // the planet is inside the disk, where it scales by afac) and its
+            // k-derivative (0 inside); bit u * TPL + j of `use` marks a value in the
+            // window with g <= 1
             int vi[NV], off[NV][TPL];
-            Real rb[NV][TPL], ra[NV][TPL], c1[NV][TPL], c2[NV][TPL], c3[NV][TPL];
-            bool use[NV][TPL], act[NV];
+            Real rb[NV][TPL], ra[NV][TPL], as[NV][TPL], dk[NV][TPL];
+            unsigned use = 0u;
+            bool act[NV];
 #pragma unroll
             for (int u = 0; u < NV; ++u) {
                 const int slot = threadIdx.x + u * kTsRowThreads;
                 vi[u] = slot + head < nv ? slot + head : slot + head - nv;
                 bool any = false;
+#pragma unroll
+                for (int j = 0; j < TPL; ++j) {
+                    off[u][j] = 0;
+                    ra[u][j] = rb[u][j] = as[u][j] = dk[u][j] = Real(0);
+                }
                 if (slot < nv) {
                     const int4 pk4 = *(const int4*)(t_pk + TPL * vi[u]);
                     const f32x4_t ra4 = *(const f32x4_t*)(t_ra + TPL * vi[u]);
@@ -827,26 +862,18 @@ __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_aligned_kernel(TsA
 #pragma unroll
                     for (int j = 0; j < TPL; ++j) {
                         const bool inside = (pk[j] & 2) != 0;
-                        use[u][j] = (pk[j] & 1) != 0;
+                        if (pk[j] & 1) use |= 1u << (u * TPL + j);
+                        any = any || (pk[j] & 1) != 0;
                         off[u][j] = pk[j] >> 2;
                         ra[u][j] = ra4[j];
                         rb[u][j] = rb4[j];
-                        c1[u][j] = inside ? a04[j] : Real(0);
-                        c2[u][j] = inside ? Real(0) : a04[j];
-                        c3[u][j] = inside ? Real(0) : dk4[j];
-                        any = any || use[u][j];
-                    }
-                } else {
-#pragma unroll
-                    for (int j = 0; j < TPL; ++j) {
-                        use[u][j] = false;
-                        off[u][j] = 0;
-                        ra[u][j] = rb[u][j] = c1[u][j] = c2[u][j] = c3[u][j] = Real(0);
+                        as[u][j] = inside ? -a04[j] : a04[j];
+                        dk[u][j] = inside ? Real(0) : dk4[j];
                     }
                 }
                 act[u] = __ballot(any) != 0ull;
             }
-#pragma unroll 2
+#pragma unroll 1
             for (int q = q0; q < pb0 + npc; q += P) {
                 const int ql = q - pb0;
                 const Real f = s_iist[ql], afac = s_afac[ql], kd = s_kd[ql];
@@ -862,8 +889,10 @@ __global__ void __launch_bounds__(kTsRowThreads) ts_flux_rows_aligned_kernel(TsA
                             for (int j = 0; j < TPL; ++j) {
                                 const Real* l = lq + off[u][j];
                                 const Real ip = rb[u][j] * l[0] + ra[u][j] * l[kTsRowLd];
-                                const Real area = c1[u][j] * afac + (c2[u][j] + c3[u][j] * kd);
-                                w[j] = use[u][j] ? Real(1) - (ip * area) * f : Real(1);
+                                // the row form's c1 afac + (c2 + c3 (k - k_mean)), with c1 = A0
+                                // inside and c2 = A0, c3 = 2 k_mean kappa_0 otherwise
+                                const Real area = as[u][j] < Real(0) ? (-as[u][j]) * afac : dk[u][j] * kd + as[u][j];
+                                w[j] = ((use >> (u * TPL + j)) & 1u) ? Real(1) - (ip * area) * f : Real(1);
                             }
                         }
                         __builtin_nontemporal_store(w, (f32x4_t*)(o + (size_t)vi[u] * TPL));
@@ -1135,7 +1164,7 @@ int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
     const int uns = lc->uniform_nsamples;
     // one sample, rows of whole 16-byte vectors: the row form
     if (uns == 1 && ((uintptr_t)flux % 16) == 0) {
-        if (mixed && npt % 4 == 0 && npt <= ts_row_max_npt<float>() &&
+        if (mixed && npt % 4 == 0 && npt <= ts_row_max_npt<float>() && kTsRowPc * a.ng <= kTsStageRegs * kTsRowThreads &&
             ts_rows_aligned_lds(a.ng, npt) <= kTsRowsAlignedMaxLds) {
             hipLaunchKernelGGL(ts_flux_rows_aligned_kernel, dim3((unsigned)npv), dim3(kTsRowThreads),
                                ts_rows_aligned_lds(a.ng, npt), st, a, (float*)flux);
