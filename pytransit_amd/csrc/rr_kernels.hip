@@ -1207,6 +1207,16 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     };
     int count = 0;
     bool ranged = false;  // the queue came from window_edges
+    // the ones around a searched window are written after the window's points:
+    // phase B's loads would otherwise wait for these stores (one memory counter
+    // covers loads and stores)
+    int fr0 = 0, fr1 = 0, fr2 = 0, fr3 = 0, fcnt = 0;
+    auto fill_deferred = [&]() {
+        if (!ranged) return;
+        wave_fill<Real>(out, tbase, fr0, Real(1), wl);
+        wave_fill<Real>(out, tbase + fr1, fr2 - fr1, Real(1), wl);
+        wave_fill<Real>(out, tbase + fr3, fcnt - fr3, Real(1), wl);
+    };
     auto push = [&](bool pred, int it) {
         const unsigned long long mk = __ballot(pred);
         if (pred) s_q[count + __popcll(mk & ((1ull << wl) - 1ull))] = (uint16_t)(it - tbase);
@@ -1243,9 +1253,11 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
                 if (window_edges(f.times, tbase, cnt, c, p, ea, eb, wl, edge)) {
                     const int r0 = edge[0], r1 = edge[1];
                     const int r2 = ea == eb ? r1 : edge[2], r3 = ea == eb ? r1 : edge[3];
-                    wave_fill<Real>(out, tbase, r0, Real(1), wl);
-                    wave_fill<Real>(out, tbase + r1, r2 - r1, Real(1), wl);
-                    wave_fill<Real>(out, tbase + r3, cnt - r3, Real(1), wl);
+                    fr0 = r0;
+                    fr1 = r1;
+                    fr2 = r2;
+                    fr3 = r3;
+                    fcnt = cnt;
                     const int n0 = r1 - r0;
                     count = n0 + (r3 - r2);
                     for (int q = wl; q < count; q += kThreads) s_q[q] = (uint16_t)(q < n0 ? r0 + q : r2 + (q - n0));
@@ -1332,7 +1344,10 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
         }
     }
     const int n = count;
-    if (n == 0) return;
+    if (n == 0) {
+        fill_deferred();
+        return;
+    }
     // one light curve with one nsamples: the first chunk's times are loaded here, in
     // flight together with the limb-darkening row staged below, and each chunk loads
     // the next chunk's times before its items (no exposed load latency per chunk)
@@ -1466,6 +1481,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
             }
             __syncthreads();
         }
+        fill_deferred();
         return;
     }
     if (kMixed && f.uniform_ns > 0 && f.nlc > 1) {
@@ -1544,6 +1560,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
             }
             __syncthreads();
         }
+        fill_deferred();
         return;
     }
     // general case: one queued point per lane, per-point light-curve context
@@ -1572,6 +1589,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
             out[it] = Real(acc / c.ns);
         }
     }
+    fill_deferred();
 }
 
 // Work items of the XCD-aware order go through the pvs in groups of kTileGroup:
