@@ -489,6 +489,21 @@ __global__ void __launch_bounds__(256) ts_ldm_f64_kernel(TsArgs s) {
     s.ldm64[idx] = d;
 }
 
+// The FP32 weights (W2T of the mixed path) when W2T does not fit the MFMA kernel's
+// LDS: one lane per (pv, passband, g), the sum in FP64, the mean stored in FP32.
+template <bool kTab>
+__global__ void __launch_bounds__(256) ts_ldm_f32_kernel(TsArgs s) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)s.npv * s.npb * s.ng) return;
+    const int ig = (int)(idx % s.ng);
+    const long long j = idx / s.ng;
+    const int pv = (int)(j / s.npb), pb = (int)(j - (long long)pv * s.npb);
+    const float* w = s.w2_32 + (size_t)pv * s.nzp * s.ngp + ig;
+    double d = 0.0;
+    for (int iz = 0; iz < s.nz; ++iz) d += w[(size_t)iz * s.ngp] * ts_profile<kTab>(s, pv, pb, iz);
+    s.ldm32[idx] = (float)d;
+}
+
 // ---- flux --------------------------------------------------------------------------
 constexpr int kTsThreads = 256;
 
@@ -1136,10 +1151,17 @@ int ts_evaluate(int precision, const rr_table* tab, const rr_lcdata* lc, const r
         if ((rc = check_launch("ts_ldm_tab_mfma_kernel"))) return rc;
     } else if (mixed) {
         const size_t lds = (size_t)a.nzp * (a.ngp + 1) * sizeof(float);
-        if (lds > 160 * 1024) return set_error(RR_ERANGE, "ts_evaluate: nz*ng too large for the MFMA kernel");
-        if (tab_ld) hipLaunchKernelGGL(ts_ldm_mfma_kernel<true>, dim3(npv), dim3(256), lds, st, a);
-        else hipLaunchKernelGGL(ts_ldm_mfma_kernel<false>, dim3(npv), dim3(256), lds, st, a);
-        if ((rc = check_launch("ts_ldm_mfma_kernel"))) return rc;
+        if (lds <= 160 * 1024) {
+            if (tab_ld) hipLaunchKernelGGL(ts_ldm_mfma_kernel<true>, dim3(npv), dim3(256), lds, st, a);
+            else hipLaunchKernelGGL(ts_ldm_mfma_kernel<false>, dim3(npv), dim3(256), lds, st, a);
+            if ((rc = check_launch("ts_ldm_mfma_kernel"))) return rc;
+        } else {
+            const long long n = (long long)npv * npb * a.ng;
+            const dim3 g((unsigned)((n + 255) / 256));
+            if (tab_ld) hipLaunchKernelGGL(ts_ldm_f32_kernel<true>, g, dim3(256), 0, st, a);
+            else hipLaunchKernelGGL(ts_ldm_f32_kernel<false>, g, dim3(256), 0, st, a);
+            if ((rc = check_launch("ts_ldm_f32_kernel"))) return rc;
+        }
     } else {
         const size_t lds = (size_t)a.nzp * (a.ngp + 1) * sizeof(double);
         if (lds <= kMfma64MaxLds) {

@@ -19,8 +19,8 @@ def say(*a):
     print(*a, flush=True)
 
 
-def ts_case(oracle, npv=4, npt=1500, npb=64, seed=2):
-    ze, zm = oracle.create_z_grid(0.7, 64, 64)
+def ts_case(oracle, npv=4, npt=1500, npb=64, seed=2, nzin=64, nzlimb=64):
+    ze, zm = oracle.create_z_grid(0.7, nzin, nzlimb)
     mu = oracle.mu_nodes(zm)
     wl = workloads.ts_many_pb(npv=npv, npt=npt, npb=npb, seed=seed, mu=mu)
     return wl, ze, mu
@@ -127,6 +127,23 @@ def test_ts_wide_g_grid_and_two_samples(gpu, oracle, prec, tol, nss):
     err = np.nanmax(np.abs(fg - fo))
     say(f'  max |dF| = {err:.3e}')
     assert (fo < 1).sum() > 10000
+    assert err <= tol
+
+
+@pytest.mark.parametrize('prec,tol', [('fp64', FP64_TOL), ('mixed', MIXED_TOL)])
+def test_ts_large_z_grid(gpu, oracle, prec, tol):
+    # 400 mu nodes: the exact-weight kernel takes 5 g rows per workgroup (16 up to
+    # 128 nodes), and W2T (400 x ng) exceeds the MFMA contractions' LDS, so both
+    # precisions contract on the vector ALUs
+    say(f'case: TS 3 pv x 24 pb x 600 t, 400 mu nodes, tabulated LD, {prec}')
+    wl, ze, mu = ts_case(oracle, npv=3, npt=600, npb=24, seed=4, nzin=200, nzlimb=200)
+    fo = oracle_ts(oracle, wl, ze)
+    tm = TSModel(TabulatedLDModel(wl.ldp, wl.istar), nzin=200, nzlimb=200, precision=prec)
+    tm.set_data(wl.times)
+    fg = np.asarray(tm.evaluate(wl.k, None, wl.t0, wl.p, wl.a, wl.i, wl.e, wl.w), float)
+    err = np.nanmax(np.abs(fg - fo))
+    say(f'  max |dF| = {err:.3e}, in-transit points {(fo < 1).sum()}')
+    assert (fo < 1).sum() > 1000
     assert err <= tol
 
 
