@@ -864,7 +864,11 @@ struct FluxArgs {
     int chunk_pts;                    // points per phase-B chunk (uniform_ns > 0)
     int row_lds;                      // stage the limb-darkening row in LDS (one light curve)
     int one_window;                   // several light curves, times sorted across them, one window
+    int npv;                          // pvs of the launch (or of the batch request)
+    int k_grp, k_last;                // the shifts of the two division multipliers below
     double dg;
+    unsigned long long m_grp;         // n / (kTileGroup * ntiles) = (n * m_grp) >> k_grp
+    unsigned long long m_last;        // n / (size of the last pv group) = (n * m_last) >> k_last
 };
 
 constexpr int kThreads = 64;
@@ -1153,7 +1157,7 @@ __device__ __forceinline__ bool window_edges(const double* times, int tbase, int
 }
 
 template <bool kMixed, bool kFast>
-__device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
+__device__ __forceinline__ void flux_body(const FluxArgs& f, int pv, int tile) {
     using Real = typename std::conditional<kMixed, float, double>::type;
     extern __shared__ __align__(16) unsigned char lds[];
     uint16_t* s_q = (uint16_t*)lds;
@@ -1161,8 +1165,6 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
     Real* s_dep = (Real*)(s_tc + kMaxChunk);
     Real* s_row = s_dep + (f.uniform_ns > 0 ? f.chunk_pts * f.uniform_ns : 0);
 
-    const int pv = bid / f.ntiles;
-    const int tile = bid - pv * f.ntiles;
     const int wl = threadIdx.x;
     const int tbase = tile * kTile;
     int its[kRuns], nrun[kRuns];
@@ -1600,19 +1602,33 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int bid) {
 // touches every pv's record and limb-darkening row).
 constexpr int kTileGroup = 256;
 
-// (pv, tile) block index of work item `item` of `nitems` = npv x ntiles.
-__device__ __forceinline__ int grouped_tile(int item, int nitems, int ntiles) {
-    const int npv = nitems / ntiles;
-    const int per = kTileGroup * ntiles;
-    const int grp = item / per, rem = item - grp * per;
-    const int gsz = min(kTileGroup, npv - grp * kTileGroup);   // the last group may be smaller
-    const int tile = rem / gsz, pv = grp * kTileGroup + (rem - tile * gsz);
-    return pv * ntiles + tile;
+// n / d for 0 <= n < 2^31 by the host's multiplier m = ceil(2^k / d), k = 31 + ceil(log2 d):
+// n m / 2^k exceeds n / d by less than n / 2^k <= 1 / d, so the floor is exact.
+__device__ __forceinline__ int udiv31(int n, unsigned long long m, int k) {
+    return (int)(((unsigned long long)(unsigned)n * m) >> k);
+}
+
+static inline void udiv31_make(int d, unsigned long long& m, int& k) {
+    int l = 0;
+    while ((1ll << l) < (long long)d) ++l;
+    k = 31 + l;
+    m = (unsigned long long)(((1ull << k) + (unsigned long long)d - 1ull) / (unsigned long long)d);
+}
+
+// (pv, tile) of work item `item` of npv x ntiles, in groups of kTileGroup pvs.
+__device__ __forceinline__ void grouped_tile(const FluxArgs& f, int item, int& pv, int& tile) {
+    const int per = kTileGroup * f.ntiles;
+    const int grp = udiv31(item, f.m_grp, f.k_grp), rem = item - grp * per;
+    const int gsz = min(kTileGroup, f.npv - grp * kTileGroup);   // the last group may be smaller
+    tile = gsz == kTileGroup ? rem / kTileGroup : udiv31(rem, f.m_last, f.k_last);
+    pv = grp * kTileGroup + (rem - tile * gsz);
 }
 
 template <bool kMixed, bool kFast>
 __global__ void __launch_bounds__(kThreads) rr_flux_kernel(FluxArgs f) {
-    flux_body<kMixed, kFast>(f, grouped_tile(xcd_remap(blockIdx.x, gridDim.x), gridDim.x, f.ntiles));
+    int pv, tile;
+    grouped_tile(f, xcd_remap(blockIdx.x, gridDim.x), pv, tile);
+    flux_body<kMixed, kFast>(f, pv, tile);
 }
 
 struct FluxBatch {
@@ -1625,7 +1641,9 @@ template <bool kMixed, bool kFast>
 __global__ void __launch_bounds__(kThreads) rr_flux_batch_kernel(FluxBatch b) {
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int i = batch_item(b.start, b.n, bid);
-    flux_body<kMixed, kFast>(b.a[i], grouped_tile(bid - b.start[i], b.start[i + 1] - b.start[i], b.a[i].ntiles));
+    int pv, tile;
+    grouped_tile(b.a[i], bid - b.start[i], pv, tile);
+    flux_body<kMixed, kFast>(b.a[i], pv, tile);
 }
 
 // Device-memory batch (see rr_setup_batch_dev_kernel).  The XCD remap is over the
@@ -1639,7 +1657,9 @@ __global__ void __launch_bounds__(kThreads) rr_flux_batch_dev_kernel(const FluxB
     if ((int)blockIdx.x >= total) return;
     const int bid = xcd_remap(blockIdx.x, total);
     const int i = batch_item(b->start, n, bid);
-    flux_body<kMixed, kFast>(b->a[i], grouped_tile(bid - b->start[i], b->start[i + 1] - b->start[i], b->a[i].ntiles));
+    int pv, tile;
+    grouped_tile(b->a[i], bid - b->start[i], pv, tile);
+    flux_body<kMixed, kFast>(b->a[i], pv, tile);
 }
 
 // A batch as the replayable launch reads it from device memory.
@@ -1891,6 +1911,9 @@ static int prepare_flux(const char* who, int precision, const rr_table* tab, con
     fa.ntiles = (npt + kTile - 1) / kTile;
     nblocks = (long long)fa.ntiles * npv;
     if (nblocks > 0x7fffffffLL) return set_error(RR_ERANGE, "%s: grid too large", who);
+    fa.npv = npv;
+    udiv31_make(kTileGroup * fa.ntiles, fa.m_grp, fa.k_grp);
+    udiv31_make(npv - (npv - 1) / kTileGroup * kTileGroup, fa.m_last, fa.k_last);
     fast = (lc->sorted & RR_SORTED_RUNS) != 0;
     fa.one_window = nlc > 1 && (lc->sorted & (RR_SORTED_RUNS | RR_SORTED_TIMES | RR_ONE_WINDOW)) ==
                                    (RR_SORTED_RUNS | RR_SORTED_TIMES | RR_ONE_WINDOW);
