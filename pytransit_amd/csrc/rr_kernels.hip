@@ -1095,6 +1095,13 @@ __device__ __forceinline__ int queue_partial(bool part, int i, T z, int pq, uint
     return pq + __popcll(mk);
 }
 
+// A tile whose first and last points (phases tca <= tcb of epochs ea <= eb) lie in
+// one sorted light curve holds no point of the window [lo, hi] when both lie before
+// or after the window of one epoch, or after one window and before the next.
+__device__ __forceinline__ bool tile_outside(double tca, double tcb, double ea, double eb, const PointCtx& c) {
+    return (ea == eb) ? (tcb < c.lo || tca > c.hi) : ((eb == ea + 1.0) ? (tca > c.hi && tcb < c.lo) : false);
+}
+
 // In-window index ranges of a tile that lies in one light curve sorted in time.
 // For the window of epoch e, tc = t - (t0 + e p) is monotone in the point index, so
 // the window is the index range [first tc >= lo, first tc > hi).  The two edges of
@@ -1182,11 +1189,24 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int pv, int tile) {
     }
     const PvRec& r = f.rec[pv];
     Real* out = (Real*)f.flux + (size_t)pv * f.npt;
-    if (!(r.good && r.ldok)) {
-        wave_fill<Real>(out, tbase, min(kTile, f.npt - tbase), Real(NAN), wl);
+    // The tile test of a tile in light curve 0's window (one light curve, or one
+    // window shared by all), computed ahead of any branch so that its loads (the
+    // record, light curve 0's context, the tile's first and last times) are all in
+    // flight together; a pv without a valid record gets NaN.
+    const int tlast = min(tbase + kTile, f.npt) - 1;
+    const double t_first = f.times[tbase], t_last = f.times[tlast];
+    const bool rec_ok = (r.good != 0) & (r.ldok != 0);
+    const PointCtx ctx0 = point_ctx0(f, r);
+    const double p = r.p, inv_p = r.inv_p;
+    double ea0, eb0;
+    const double tca0 = phase_of(t_first, ctx0.t0v, p, inv_p, ea0);
+    const double tcb0 = phase_of(t_last, ctx0.t0v, p, inv_p, eb0);
+    const bool one_ctx = kFast && (f.nlc == 1 || f.one_window);
+    const bool skip = (!rec_ok) | (one_ctx & tile_outside(tca0, tcb0, ea0, eb0, ctx0));  // no short circuit
+    if (skip) {
+        wave_fill<Real>(out, tbase, tlast + 1 - tbase, rec_ok ? Real(1) : Real(NAN), wl);
         return;
     }
-    const double p = r.p, inv_p = r.inv_p;
     // sorted light curves: the tile's points lie in light curves la0..lb0
     // (lcids are non-decreasing); up to kTileLcs of them are staged in LDS
     // (staged once the tile is known to hold a window)
@@ -1232,21 +1252,23 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int pv, int tile) {
         // The tile as a whole first: most tiles of a long light curve hold no point
         // of a transit window and are filled with ones at once.
         {
-            const int last = min(tbase + kTile, f.npt) - 1;
-            const int la = f.nlc == 1 || f.one_window ? 0 : min(max(f.lcids[tbase], 0), f.nlc - 1);
-            const int lb = f.nlc == 1 || f.one_window ? 0 : min(max(f.lcids[last], 0), f.nlc - 1);
+            const int last = tlast;
+            const int la = one_ctx ? 0 : min(max(f.lcids[tbase], 0), f.nlc - 1);
+            const int lb = one_ctx ? 0 : min(max(f.lcids[last], 0), f.nlc - 1);
             if (la == lb) {
                 // the window of the tile's light curve, or the one window all
-                // light curves share (times then sorted across light curves too)
-                const PointCtx c = f.nlc == 1 || f.one_window ? point_ctx0(f, r) : point_ctx(f, r, pv, la);
-                double ea, eb;
-                const double tca = phase_of(f.times[tbase], c.t0v, p, inv_p, ea);
-                const double tcb = phase_of(f.times[last], c.t0v, p, inv_p, eb);
-                const bool out_tile = (ea == eb) ? (tcb < c.lo || tca > c.hi)
-                                                 : ((eb == ea + 1.0) ? (tca > c.hi && tcb < c.lo) : false);
-                if (out_tile) {
-                    wave_fill<Real>(out, tbase, last + 1 - tbase, Real(1), wl);
-                    return;
+                // light curves share (times then sorted across light curves too;
+                // tested above)
+                PointCtx c = ctx0;
+                double ea = ea0, eb = eb0;
+                if (!one_ctx) {
+                    c = point_ctx(f, r, pv, la);
+                    const double tca = phase_of(t_first, c.t0v, p, inv_p, ea);
+                    const double tcb = phase_of(t_last, c.t0v, p, inv_p, eb);
+                    if (tile_outside(tca, tcb, ea, eb, c)) {
+                        wave_fill<Real>(out, tbase, last + 1 - tbase, Real(1), wl);
+                        return;
+                    }
                 }
                 // the window's index ranges by search: ones around them, the
                 // ranges into the queue, no run classification
@@ -1268,7 +1290,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int pv, int tile) {
             }
         }
         stage_lcs();
-        const PointCtx c1 = f.nlc == 1 ? point_ctx0(f, r) : ctx_of(la0);
+        const PointCtx c1 = f.nlc == 1 ? ctx0 : ctx_of(la0);
 #pragma unroll
         for (int m = 0; m < (ranged ? 0 : kRuns); ++m) {
             bool all_out = true, one_lc = true, one_ep = false;
@@ -1620,7 +1642,9 @@ __device__ __forceinline__ void grouped_tile(const FluxArgs& f, int item, int& p
     const int per = kTileGroup * f.ntiles;
     const int grp = udiv31(item, f.m_grp, f.k_grp), rem = item - grp * per;
     const int gsz = min(kTileGroup, f.npv - grp * kTileGroup);   // the last group may be smaller
-    tile = gsz == kTileGroup ? rem / kTileGroup : udiv31(rem, f.m_last, f.k_last);
+    // a full group divides by kTileGroup = 2^8: multiplier 2^31, shift 39 (no branch)
+    const bool full = gsz == kTileGroup;
+    tile = udiv31(rem, full ? (1ull << 31) : f.m_last, full ? 39 : f.k_last);
     pv = grp * kTileGroup + (rem - tile * gsz);
 }
 
