@@ -854,11 +854,11 @@ rr_setup_batch_dev_kernel(const SetupBatch* __restrict__ b) {
 // and waves in phase B (VALU-bound) share each CU.
 // ------------------------------------------------------------------------------------
 struct FluxArgs {
-    const double* times; const int32_t* lcids; const int32_t* pbids; const int32_t* epids;
-    const int32_t* nsamples; const double* exptimes;
+    // the tile test's arrays first: one 64-byte kernel-argument load holds them
+    const double* times; const PvRec* rec; void* flux; const double* exptimes;
+    const int32_t* pbids; const int32_t* nsamples; const int32_t* lcids; const int32_t* epids;
     const double* k; const double* t0; const double* istar;
-    const PvRec* rec; const double* ldm64; const float* ldm32;
-    void* flux;
+    const double* ldm64; const float* ldm32;
     int npt, nlc, npb, kcols, t0cols, ng, ntiles;
     int uniform_ns;                   // the common nsamples of all light curves, or 0
     int chunk_pts;                    // points per phase-B chunk (uniform_ns > 0)
@@ -1180,13 +1180,6 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int pv, int tile) {
         its[m] = tbase + (m * kThreads + wl) * kRun;
         nrun[m] = min(kRun, f.npt - its[m]);  // <= 0 past the end
     }
-    // issue the time loads before anything depends on the per-pv record
-    double tf[kRuns], tl[kRuns];
-#pragma unroll
-    for (int m = 0; m < kRuns; ++m) {
-        tf[m] = nrun[m] > 0 ? f.times[its[m]] : 0.0;
-        tl[m] = nrun[m] > 0 ? f.times[its[m] + nrun[m] - 1] : 0.0;
-    }
     const PvRec& r = f.rec[pv];
     Real* out = (Real*)f.flux + (size_t)pv * f.npt;
     // The tile test of a tile in light curve 0's window (one light curve, or one
@@ -1291,6 +1284,14 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int pv, int tile) {
         }
         stage_lcs();
         const PointCtx c1 = f.nlc == 1 ? ctx0 : ctx_of(la0);
+        // the runs' first and last times (only tiles classified run by run need
+        // them, and a wave keeps its slot until its loads return)
+        double tf[kRuns], tl[kRuns];
+#pragma unroll
+        for (int m = 0; m < kRuns; ++m) {
+            tf[m] = !ranged && nrun[m] > 0 ? f.times[its[m]] : 0.0;
+            tl[m] = !ranged && nrun[m] > 0 ? f.times[its[m] + nrun[m] - 1] : 0.0;
+        }
 #pragma unroll
         for (int m = 0; m < (ranged ? 0 : kRuns); ++m) {
             bool all_out = true, one_lc = true, one_ep = false;
