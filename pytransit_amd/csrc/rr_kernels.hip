@@ -1023,29 +1023,48 @@ __device__ __forceinline__ float sample_f32(const float* cf, float kf, float inv
 }
 
 // sample_f64 in two passes for the item loop with a queue of partial overlaps:
-// returns the sample's (istar - I*A)/istar unless the planet straddles the limb,
+// returns the sample's 1 - I*A/istar unless the planet straddles the limb,
 // |1 - k| < z < 1 + k, where it returns I and sets `part`; sample_f64_finish then
-// completes it with the same operations as sample_f64.
+// completes it.  The divisions of sample_f64 (by ns, 1 + k, dg and istar) are
+// products with the light curve's reciprocals (RecipF64): each changes a result
+// by an ulp or so, and they were the long-latency links of the sample's chain.
+// Reciprocals of one light curve's per-sample divisors, for the FP64 item loop: the
+// divisions become products (a change of the last bit at most).
+struct RecipF64 {
+    double inv_1pk, inv_ist, inv_dg, inv_ns;
+};
+
+// interp_ld with g / dg as g * (1 / dg)
+__device__ __forceinline__ double interp_ld_r(double g, double dg, double inv_dg, const double* __restrict__ row, int ng) {
+    if (g < 0.0) return NAN;
+    if (g > 1.0) return 0.0;
+    int i = (int)floor(g * inv_dg);
+    i = i > ng - 2 ? ng - 2 : (i < 0 ? 0 : i);
+    const double a = (g - i * dg) * inv_dg;
+    return (1.0 - a) * row[i] + a * row[i + 1];
+}
+
 __device__ __forceinline__ double sample_f64_pass1(const double* cc, double dg, int ng, const PointCtx& c,
-                                                   const double* lrow, double tc, int is, double& z, bool& part) {
-    const double off = c.ext * ((is + 1 - 0.5) / c.ns - 0.5);
+                                                   const RecipF64& rc, const double* lrow, double tc, int is,
+                                                   double& z, bool& part) {
+    const double off = c.ext * ((is + 0.5) * rc.inv_ns - 0.5);
     z = taylor_z<double>(tc + off, cc);
     part = false;
     if (!(z < 1.0 + c.kk)) return 1.0;
-    const double ip = interp_ld(z / (1.0 + c.kk), dg, lrow, ng);
+    const double ip = interp_ld_r(z * rc.inv_1pk, dg, rc.inv_dg, lrow, ng);
     if (d_abs(1.0 - c.kk) < z) {
         part = true;
         return ip;
     }
     double ap, kap;
     ccia_kite<double>(1.0, c.kk, z, ap, kap);
-    return (c.ist - ip * ap) / c.ist;
+    return 1.0 - (ip * ap) * rc.inv_ist;
 }
 
-__device__ __forceinline__ double sample_f64_finish(double ip, const PointCtx& c, double z) {
+__device__ __forceinline__ double sample_f64_finish(double ip, const PointCtx& c, const RecipF64& rc, double z) {
     double ap, kap;
     ccia_kite<double>(1.0, c.kk, z, ap, kap);
-    return (c.ist - ip * ap) / c.ist;
+    return 1.0 - (ip * ap) * rc.inv_ist;
 }
 
 // Lens (kite) area of the unit disk and a planet of radius kf < 1 at separation z,
@@ -1411,6 +1430,13 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int pv, int tile) {
         if (kMixed && wl < ns) s_off[wl] = extf * (((float)wl + 0.5f) * inv_ns - 0.5f);
         const bool small = kMixed && kf < 1.0f && ns <= kThreads;
         const int cp = f.chunk_pts;
+        RecipF64 rc64;
+        if (!kMixed) {
+            rc64.inv_1pk = 1.0 / (1.0 + c.kk);
+            rc64.inv_ist = 1.0 / c.ist;
+            rc64.inv_dg = 1.0 / f.dg;
+            rc64.inv_ns = 1.0 / c.ns;
+        }
         for (int c0 = 0; c0 < n; c0 += cp) {
             const int cn = min(cp, n - c0);
             const double tcur = tnext;
@@ -1463,13 +1489,13 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int pv, int tile) {
                     bool part = false;
                     double z = 0.0;
                     if (i < tot)
-                        s_dep[i] = sample_f64_pass1(r.c, f.dg, f.ng, c, (const double*)s_row, s_tc[qq], is, z, part);
+                        s_dep[i] = sample_f64_pass1(r.c, f.dg, f.ng, c, rc64, (const double*)s_row, s_tc[qq], is, z, part);
                     pq = queue_partial<Real>(part, i, z, pq, s_pi, s_pz, wl);
                     if (pq >= kThreads) {
                         __syncthreads();
                         pq -= kThreads;
                         const int j = s_pi[pq + wl];
-                        s_dep[j] = sample_f64_finish(s_dep[j], c, s_pz[pq + wl]);
+                        s_dep[j] = sample_f64_finish(s_dep[j], c, rc64, s_pz[pq + wl]);
                         __syncthreads();
                     }
                     qq += dq;
@@ -1482,7 +1508,7 @@ __device__ __forceinline__ void flux_body(const FluxArgs& f, int pv, int tile) {
                 __syncthreads();
                 if (wl < pq) {
                     const int j = s_pi[wl];
-                    s_dep[j] = sample_f64_finish(s_dep[j], c, s_pz[wl]);
+                    s_dep[j] = sample_f64_finish(s_dep[j], c, rc64, s_pz[wl]);
                 }
             } else {
 #pragma unroll 1
