@@ -222,6 +222,8 @@ __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
     const Real pik2 = (Real)(kPi * kk * kk), dpik2 = (Real)(2.0 * kPi * kk);
     const double scale = 1.0 / (ns * s_ist);
     const Real rscale = (Real)scale;
+    // per-sample and per-point divisors as reciprocals (off the samples' chains)
+    const double inv_ns = 1.0 / ns, scale_ist = scale / s_ist;
     double chi = 0.0, g[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int it = tid; it < s.npt; it += kLnlThreads) {
         const double t = s.times[it];
@@ -244,7 +246,7 @@ __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
         Real D = 0, dDk = 0, dDt0 = 0, dDp = 0, dDa = 0, dDi = 0, dDc0 = 0, dDc1 = 0;
         const Real ep = (Real)epoch;
         for (int is = 1; is <= ns; ++is) {
-            const double taud = tc + ext * ((is - 0.5) / ns - 0.5);
+            const double taud = tc + ext * ((is - 0.5) * inv_ns - 0.5);
             const Real tau = (Real)taud;
             Real x, y, z, u;
             if constexpr (kMixed) {
@@ -293,6 +295,7 @@ __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
                 Ic0 = s_Lc0[i] + al * (s_Lc0[i + 1] - s_Lc0[i]);
                 Ic1 = s_Lc1[i] + al * (s_Lc1[i + 1] - s_Lc1[i]);
             }
+            const Real inv_z = Real(1) / z;
             // area and its slopes
             Real A, dAdz, dAdk;
             if (z <= Real(1) - kr) {
@@ -305,11 +308,10 @@ __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
                 } else {
                     ccia_kite3<Real>(Real(1), kr, z, A, kap, akite);
                 }
-                dAdz = Real(-2) * akite / z;   // minus the chord of the lens
+                dAdz = Real(-2) * akite * inv_z;   // minus the chord of the lens
                 dAdk = Real(2) * kr * kap;
             }
             // z derivatives
-            const Real inv_z = Real(1) / z;
             const Real dz_t0 = -(x * vx + y * vy) * inv_z;
             const Real dz_p = (x * (px - ep * vx) + y * (py - ep * vy)) * inv_z;
             const Real dz_a = z * inv_a;
@@ -332,8 +334,8 @@ __global__ void __launch_bounds__(kLnlThreads) rr_lnlike_kernel(LnlArgs s) {
         g[2] += rs * (double)dDp;
         g[3] += rs * (double)dDa;
         g[4] += rs * (double)dDi;
-        g[5] += rs * (double)dDc0 + r * (double)D * scale / s_ist * s_dist0;
-        g[6] += rs * (double)dDc1 + r * (double)D * scale / s_ist * s_dist1;
+        g[5] += rs * (double)dDc0 + r * (double)D * scale_ist * s_dist0;
+        g[6] += rs * (double)dDc1 + r * (double)D * scale_ist * s_dist1;
     }
     // ---- block reduction ----------------------------------------------------------
     double v[8] = {chi, g[0], g[1], g[2], g[3], g[4], g[5], g[6]};
