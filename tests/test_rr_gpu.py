@@ -124,15 +124,18 @@ def test_rr_small_mixed(gpu, oracle):
     assert err <= MIXED_TOL
 
 
-def test_tile_groups_with_a_partial_last_group(gpu, oracle):
+@pytest.mark.parametrize('npv', [300, 257, 1])
+def test_tile_groups_with_a_partial_last_group(gpu, oracle, npv):
     # The flux kernel walks the pvs in groups of 256 (within a group the same time
-    # tile of every pv first): 300 pv x 3000 t (3 tiles) is one full group and one
-    # of 44, whose tiles are dealt with the smaller group's stride.
-    say('case: 300 pv x 3000 t, nss=3, quadratic, fp64 and mixed (tile groups of 256 and 44)')
-    wl = workloads.rr_small(npv=300, npt=3000, seed=7)
+    # tile of every pv first), dividing a block index by the group's size with a
+    # host-computed multiplier: 300 pv x 3000 t (3 tiles) is one full group and one
+    # of 44; 257 pv a full group and one of a single pv (divisor 1); 1 pv one group
+    # of one.
+    say(f'case: {npv} pv x 3000 t, nss=3, quadratic, fp64 and mixed (tile groups of 256 and {npv % 256})')
+    wl = workloads.rr_small(npv=npv, npt=3000, seed=7)
     wl.nsamples, wl.exptimes = 3, 0.02
     fo = oracle_flux(oracle, wl, 'quadratic')
-    assert (fo < 1).sum() > 10000
+    assert (fo < 1).sum() > 30 * npv
     for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
         fg = gpu_flux(wl, prec)
         assert np.array_equal(np.isnan(fo), np.isnan(fg)), prec
