@@ -144,6 +144,24 @@ def test_tile_groups_with_a_partial_last_group(gpu, oracle, npv):
         assert err <= tol
 
 
+@pytest.mark.parametrize('nss', [25, 64])
+def test_many_samples_small_chunks(gpu, oracle, nss):
+    # 1024 (point, sample) items per phase-B chunk: nss = 25 gives chunks of 40
+    # points, nss = 64 of 16 (the FP32 small-planet loop's limit), so a window's
+    # points span several chunks of fewer than 64
+    say(f'case: 40 pv x 3000 t, nss={nss} over 30 min, quadratic, fp64 and mixed')
+    wl = workloads.rr_small(npv=40, npt=3000, seed=9)
+    wl.nsamples, wl.exptimes = nss, 30.0 / 1440.0
+    fo = oracle_flux(oracle, wl, 'quadratic')
+    assert (fo < 1).sum() > 40 * 30
+    for prec, tol in (('fp64', FP64_TOL), ('mixed', MIXED_TOL)):
+        fg = gpu_flux(wl, prec)
+        assert np.array_equal(np.isnan(fo), np.isnan(fg)), prec
+        err = np.nanmax(np.abs(fg - fo))
+        say(f'  {prec}: max |dF| = {err:.3e}')
+        assert err <= tol
+
+
 @pytest.mark.parametrize('stress', [False, True])
 def test_rr_bjd_slice(gpu, oracle, stress):
     say(f'case: rr_bjd slice 24x20000 power-2 nss=10 stress={stress}')
